@@ -1,0 +1,166 @@
+/*
+ * teufel_gpu.h - C ABI of the MI355X (gfx950) Lienard-Wiechert radiation
+ * library: the post-tracking radiation step of TEUFEL on one GPU.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.  No C++
+ * exceptions and no framework types cross this boundary.  All entry points are
+ * callable from one host thread per context (the reference calls
+ * Observer::integrate() from one thread, src/teufel_hybrid.cpp:708); distinct
+ * contexts may be used concurrently.
+ *
+ * Units and layouts follow the reference:
+ *   trajectories  SoA, particle-major, n_points per particle:
+ *                 t [s], x,y,z [m], ux,uy,uz = beta*gamma, ax,ay,az =
+ *                 d(beta*gamma)/dt [1/s] exactly as ChargedParticle stores them
+ *                 (src/particle.h:417-427); charge per particle in units of e.
+ *   traces        [n_pix][n_bins][6] doubles (Ex,Ey,Ez [V/m], Bx,By,Bz [T]),
+ *                 the getBuffer layout of src/mesh.cpp:375-399.  Sample i of a
+ *                 pixel is the average field over [t0+(i-1/2)dt, t0+(i+1/2)dt).
+ *
+ * Semantics kept from the reference:
+ *   - integration accumulates (+=) into the device trace store; tg_zero_screen
+ *     clears it (MeshedScreen::zero, src/mesh.cpp:281-286);
+ *   - a screen with n_bins < 2 and trajectories with n_points < 2 are silent
+ *     no-ops (src/particle.cpp:778,783);
+ *   - bucket indices are round-half-away-from-zero of (t_ret - t0)/dt and a
+ *     step contributes only if t_ret2 > tmin && t_ret1 < tmax
+ *     (src/particle.cpp:793,805,829-830).
+ */
+#ifndef TEUFEL_GPU_H
+#define TEUFEL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (0 = success).  The host adapter maps non-zero codes onto the
+ * reference's IOexception (src/global.h:78-96). */
+#define TG_OK 0
+#define TG_ERR_ARG 1   /* invalid argument (null pointer, bad size, dt <= 0, non-finite) */
+#define TG_ERR_OOM 2   /* device or host allocation failed */
+#define TG_ERR_HIP 3   /* a HIP runtime call failed */
+#define TG_ERR_RANGE 4 /* index out of range (screen id, particle range) */
+#define TG_ERR_STATE 5 /* call sequence error (e.g. integrate before trajectories) */
+
+/* precision of tg_integrate */
+#define TG_FP64 0  /* plain FP64; timing arithmetic bit-compatible with the reference */
+#define TG_MIXED 1 /* FP64 retarded time and 1-n.beta, FP32 field amplitudes, FP64 traces */
+
+typedef struct tg_context tg_context;
+
+/* Launch shape of the radiation kernel; zero fields mean "autotuned/default". */
+typedef struct {
+    int bins_per_window;     /* trace bins held in LDS per work item: 4, 8, 16 or 32 */
+    int particles_per_chunk; /* particles per work item (partial traces summed in order) */
+    int use_graph;           /* 1: replay the launch sequence from a captured hipGraph */
+    int reserved;
+} tg_tuning;
+
+/* Context on one device; one trace store and one trajectory set per context.
+ * Replaces the per-rank observer loop + MPI_Reduce of
+ * src/teufel_hybrid.cpp:691-763 with an on-device store. */
+int tg_create(int device, tg_context** ctx);
+int tg_destroy(tg_context* ctx);
+/* Text of the last error on this context ("" if none); valid until the next call. */
+const char* tg_last_error(const tg_context* ctx);
+/* Device name and compute unit count, for reports. */
+int tg_device_info(const tg_context* ctx, char* name, int name_len, int* n_cu);
+
+/* Trajectories of all particles (Beam order: bunches, then particles; the
+ * summation order of Beam/Bunch::integrateFieldTrace, src/beam.cpp:443-450,
+ * src/bunch.cpp:739-745).  Host arrays of n_particles*n_points doubles,
+ * copied; the caller keeps ownership.  Replaces the particle trajectory store
+ * read by ChargedParticle::integrateFieldTrace (src/particle.cpp:771-867). */
+int tg_set_trajectories(tg_context* ctx, int n_particles, int n_points,
+                        const double* t, const double* x, const double* y, const double* z,
+                        const double* ux, const double* uy, const double* uz,
+                        const double* ax, const double* ay, const double* az,
+                        const double* charge);
+
+/* Same from device-resident arrays (e.g. produced by tg_track_undulator or by
+ * the caller's own kernels); copied, the caller keeps ownership. */
+int tg_set_trajectories_device(tg_context* ctx, int n_particles, int n_points,
+                               const double* t, const double* x, const double* y, const double* z,
+                               const double* ux, const double* uy, const double* uz,
+                               const double* ax, const double* ay, const double* az,
+                               const double* charge);
+
+/* A screen: n_pix observation points obs[n_pix][3] with per-pixel t0[n_pix],
+ * common dt and n_bins.  Mirrors the FieldTrace(t0,dt,N) per pixel of
+ * MeshedScreen (src/mesh.cpp:169-175), ScreenObserver (src/screen.cpp:74-88)
+ * and PointObserver (src/point_observer.cpp:36-45).  The trace store starts
+ * zeroed. */
+int tg_add_screen(tg_context* ctx, int n_pix, const double* obs, const double* t0,
+                  double dt, int n_bins, int* screen_id);
+int tg_zero_screen(tg_context* ctx, int screen_id);
+int tg_remove_screens(tg_context* ctx);
+int tg_screen_count(const tg_context* ctx, int* n_screens);
+
+/* Observer::integrate() (src/observer.h:140) for every screen of the context,
+ * in one batched launch: adds the fields of all particles to every pixel's
+ * trace.  precision: TG_FP64 or TG_MIXED.  tuning may be NULL.  Returns after
+ * the work is queued on the context's stream; tg_synchronize waits for it. */
+int tg_integrate(tg_context* ctx, int precision, const tg_tuning* tuning);
+int tg_synchronize(tg_context* ctx);
+
+/* As tg_integrate for a particle subset [p_begin, p_end) and one screen
+ * (screen_id >= 0) or all screens (screen_id < 0). */
+int tg_integrate_range(tg_context* ctx, int precision, const tg_tuning* tuning,
+                       int screen_id, int p_begin, int p_end);
+
+/* Copy a screen's traces to host, out[n_pix][n_bins][6]; accumulate = 1 adds
+ * into out (FieldTrace += semantics, src/fieldtrace.cpp:185-190), 0 overwrites.
+ * Replaces getBuffer + MPI_Reduce + fromBuffer (src/mesh.cpp:375-425,
+ * src/teufel_hybrid.cpp:726-763).  Synchronizes the context's stream. */
+int tg_read_traces(tg_context* ctx, int screen_id, double* out, int accumulate);
+/* Device pointer to a screen's trace store ([n_pix][n_bins][6] doubles). */
+int tg_screen_traces_device(tg_context* ctx, int screen_id, double** traces);
+
+/* Per-pixel radiated energy density on device: S_p = dt * sum_i E_i x B_i / mu0
+ * (FieldTrace::Poynting, src/fieldtrace.cpp:192-198), out[n_pix][3] on host. */
+int tg_screen_poynting(tg_context* ctx, int screen_id, double* out);
+
+/* The bucket deposit alone, driven by injected per-point retarded times and
+ * fields, as the Special particle of tests/teufel.integrate_field.cpp:51-82:
+ * ts[n_pseudo][n_points], fields[n_pseudo][n_points][6] (host), trace
+ * [n_bins][6] (host, accumulated).  Runs the device deposit code of
+ * tg_integrate.  Replaces ChargedParticle::integrateFieldTrace with
+ * overridden RetardedTime/RetardedField (src/particle.cpp:771-867). */
+int tg_deposit_only(tg_context* ctx, int n_pseudo, int n_points, const double* ts,
+                    const double* fields, double t0, double dt, int n_bins, double* trace);
+
+/* Pick tuning for the context's current screens and trajectories by timing
+ * candidate launch shapes on the device; the result is used by later
+ * tg_integrate calls with tuning == NULL and written to `out` if non-NULL. */
+int tg_autotune(tg_context* ctx, int precision, tg_tuning* out);
+
+/* Elapsed device time (ms) of the last tg_integrate's radiation kernel and of
+ * the whole launch sequence, measured with HIP events on the context's stream. */
+int tg_last_timing(tg_context* ctx, float* kernel_ms, float* total_ms);
+
+/* Trajectory generation on the device (the tracking step that precedes the
+ * radiation step): Vay pusher (J.-L. Vay 2008) through a planar undulator,
+ * ChargedParticle::InitVay/StepVay (src/particle.cpp:407-512) with
+ * PlanarUndulator::LocalField (src/undulator.cpp:107-126) placed at origin[3].
+ * x0[n][3], u0[n][3] initial positions/momenta (host), all particles start at
+ * t_start; charge/mass in units of e / m_e; n_steps+1 points per particle
+ * become the context's trajectories (replacing any previous set). */
+int tg_track_undulator(tg_context* ctx, int n_particles, const double* x0, const double* u0,
+                       double t_start, double charge, double mass, double dt, int n_steps,
+                       double bpeak, double lambda_u, int n_periods, const double* origin);
+
+/* Copy particles [p_begin, p_end) of the context's trajectories to host SoA
+ * arrays ((p_end-p_begin)*n_points each; any pointer may be NULL). */
+int tg_get_trajectories(tg_context* ctx, int p_begin, int p_end,
+                        double* t, double* x, double* y, double* z,
+                        double* ux, double* uy, double* uz,
+                        double* ax, double* ay, double* az, double* charge);
+int tg_trajectory_shape(const tg_context* ctx, int* n_particles, int* n_points);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

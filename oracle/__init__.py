@@ -1,0 +1,267 @@
+"""ctypes access to the test-only checkers built by oracle/Makefile.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the teufel_amd product.
+
+* ``lib()``      -> oracle/_build/liblw_oracle.so, the C restatement
+                    (oracle/lw_oracle.c) of the reference radiation path.
+* ``ref_lib()``  -> oracle/_ref/libteufel_ref.so, the reference's own
+                    src/{vector,fields,fieldtrace,particle,global}.cpp plus
+                    oracle/ref_harness.cpp; None when it was not built.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liblw_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libteufel_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+_ref = None
+
+
+def build(quiet=True):
+    """Run oracle/Makefile (builds _ref only where the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        i, d = ctypes.c_int, ctypes.c_double
+        L.or_retarded_time.restype = d
+        L.or_deposit_step.argtypes = [d, d, _dp, _dp, d, d, i, _dp]
+        L.or_integrate_pseudo.argtypes = [i, _dp, _dp, d, d, i, _dp]
+        L.or_integrate_screen.argtypes = [ctypes.c_void_p, i, _dp, _dp, d, i, _dp, i]
+        L.or_screen_cells.argtypes = [_dp, _dp, _dp, i, i, d, d, d, _dp, _dp]
+        L.or_trace_poynting.argtypes = [_dp, i, d, _dp]
+        L.or_total_energy.argtypes = [i, _dp, i, d, _dp, _dp]
+        L.or_total_energy.restype = d
+        L.or_undulator_field.argtypes = [d, d, i, _dp, _dp, _dp]
+        L.or_track_undulator.argtypes = [i, _dp, _dp, d, d, d, d, i, d, d, i, _dp] + [_dp] * 10 + [i]
+        L.or_retarded_field.argtypes = [ctypes.c_void_p, i, i, _dp, _dp]
+        L.or_retarded_time.argtypes = [ctypes.c_void_p, i, i, _dp]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The compiled reference, or None where it has not been built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_PATH):
+            return None
+        R = ctypes.CDLL(REF_PATH)
+        i, d, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        R.ref_beam_create.restype = vp
+        R.ref_beam_create.argtypes = [i, i] + [_dp] * 11
+        R.ref_beam_destroy.argtypes = [vp]
+        R.ref_beam_integrate.argtypes = [vp, i, i, i, _dp, _dp, d, i, _dp, i]
+        R.ref_pseudo_integrate.argtypes = [i, i, _dp, _dp, d, d, i, _dp]
+        R.ref_track_undulator.argtypes = [i, _dp, _dp, d, d, d, d, i, d, d, i, _dp] + [_dp] * 10
+        R.ref_track_homogeneous.argtypes = [_dp, _dp, _dp, _dp, d, d, d, i] + [_dp] * 10
+        R.ref_retarded_field_at.argtypes = [vp, i, d, _dp, _dp]
+        _ref = R
+    return _ref
+
+
+class Trajectories:
+    """SoA trajectories, particle-major: arrays of shape (n_particles, n_points).
+
+    Fields: t, x, y, z, ux, uy, uz (beta*gamma), ax, ay, az (d(beta*gamma)/dt),
+    charge (n_particles,), in units of e.
+    """
+    NAMES = ("t", "x", "y", "z", "ux", "uy", "uz", "ax", "ay", "az")
+
+    def __init__(self, **arrays):
+        for n in self.NAMES:
+            setattr(self, n, np.ascontiguousarray(arrays[n], dtype=np.float64))
+        self.charge = np.ascontiguousarray(arrays["charge"], dtype=np.float64)
+        self.n_particles, self.n_points = self.t.shape
+
+    @classmethod
+    def empty(cls, n_particles, n_points, charge=-1.0):
+        z = {n: np.zeros((n_particles, n_points)) for n in cls.NAMES}
+        z["charge"] = np.full(n_particles, charge, dtype=np.float64)
+        return cls(**z)
+
+    def subset(self, p0, p1):
+        d = {n: getattr(self, n)[p0:p1] for n in self.NAMES}
+        d["charge"] = self.charge[p0:p1]
+        return Trajectories(**d)
+
+    def arrays(self):
+        return [getattr(self, n) for n in self.NAMES]
+
+
+class _ORTraj(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int), ("n_points", ctypes.c_int)] + \
+               [(n, _dp) for n in Trajectories.NAMES] + [("charge", _dp)]
+
+
+def _or_struct(tr):
+    s = _ORTraj(tr.n_particles, tr.n_points, *[_ptr(a) for a in tr.arrays()], _ptr(tr.charge))
+    return s
+
+
+def integrate_screen(tr, obs, t0, dt, n_bins, traces=None, n_threads=0):
+    """MeshedScreen::integrate restated (oracle); returns traces (n_pix, n_bins, 6)."""
+    obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    t0 = np.ascontiguousarray(t0, dtype=np.float64).reshape(-1)
+    n_pix = obs.shape[0]
+    if traces is None:
+        traces = np.zeros((n_pix, max(n_bins, 0), 6))
+    s = _or_struct(tr)
+    err = lib().or_integrate_screen(ctypes.byref(s), n_pix, _ptr(obs), _ptr(t0), dt, n_bins,
+                                    _ptr(traces), n_threads)
+    if err:
+        raise IndexError("FieldTrace::add - index out of range.")
+    return traces
+
+
+def integrate_pseudo(ts, f, t0, dt, n_bins, trace=None):
+    """Deposit driven by injected (t_ret, field) per point; ts (n_pseudo, n_points),
+    f (n_pseudo, n_points, 6).  Returns trace (n_bins, 6)."""
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if ts.ndim == 1:
+        ts, f = ts[None], f[None]
+    if trace is None:
+        trace = np.zeros((n_bins, 6))
+    for p in range(ts.shape[0]):
+        err = lib().or_integrate_pseudo(ts.shape[1], _ptr(ts[p]), _ptr(f[p]), t0, dt, n_bins, _ptr(trace))
+        if err:
+            raise IndexError("FieldTrace::add - index out of range.")
+    return trace
+
+
+def retarded_field(tr, p, k, obs):
+    out = np.zeros(6)
+    s = _or_struct(tr)
+    obs = np.ascontiguousarray(obs, dtype=np.float64)
+    lib().or_retarded_field(ctypes.byref(s), p, k, _ptr(obs), _ptr(out))
+    return out
+
+
+def retarded_time(tr, p, k, obs):
+    s = _or_struct(tr)
+    obs = np.ascontiguousarray(obs, dtype=np.float64)
+    return lib().or_retarded_time(ctypes.byref(s), p, k, _ptr(obs))
+
+
+def screen_cells(origin, dX, dY, nx, ny, t0_obs, dtx, dty):
+    obs = np.zeros((nx * ny, 3))
+    t0 = np.zeros(nx * ny)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (origin, dX, dY)]
+    lib().or_screen_cells(_ptr(a[0]), _ptr(a[1]), _ptr(a[2]), nx, ny, t0_obs, dtx, dty, _ptr(obs), _ptr(t0))
+    return obs, t0
+
+
+def trace_poynting(trace, dt):
+    trace = np.ascontiguousarray(trace, dtype=np.float64)
+    S = np.zeros(3)
+    lib().or_trace_poynting(_ptr(trace), trace.shape[0], dt, _ptr(S))
+    return S
+
+
+def total_energy(traces, dt, area, normal):
+    traces = np.ascontiguousarray(traces, dtype=np.float64)
+    area = np.ascontiguousarray(area, dtype=np.float64)
+    normal = np.ascontiguousarray(normal, dtype=np.float64)
+    return lib().or_total_energy(traces.shape[0], _ptr(traces), traces.shape[1], dt, _ptr(area), _ptr(normal))
+
+
+def undulator_field(bpeak, lambda_u, n_periods, origin, X):
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    x = np.ascontiguousarray(X, dtype=np.float64)
+    B = np.zeros(3)
+    lib().or_undulator_field(bpeak, lambda_u, n_periods, _ptr(o), _ptr(x), _ptr(B))
+    return B
+
+
+def track_undulator(x0, u0, dt, n_steps, bpeak, lambda_u, n_periods, origin,
+                    charge=-1.0, mass=1.0, t_start=0.0, n_threads=0, use_ref=False):
+    """Vay pusher through a planar undulator (oracle restatement, or the
+    reference's own InitVay/StepVay with use_ref=True)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1, 3)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1, 3)
+    n = x0.shape[0]
+    tr = Trajectories.empty(n, n_steps + 1, charge)
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    outs = [_ptr(a) for a in tr.arrays()]
+    if use_ref:
+        R = ref_lib()
+        if R is None:
+            raise RuntimeError("oracle/_ref/libteufel_ref.so not built")
+        R.ref_track_undulator(n, _ptr(x0), _ptr(u0), t_start, charge, mass, dt, n_steps, bpeak,
+                              lambda_u, n_periods, _ptr(o), *outs)
+    else:
+        lib().or_track_undulator(n, _ptr(x0), _ptr(u0), t_start, charge, mass, dt, n_steps, bpeak,
+                                 lambda_u, n_periods, _ptr(o), *outs, n_threads)
+    return tr
+
+
+class RefBeam:
+    """The reference's ChargedParticle objects built from Trajectories."""
+
+    def __init__(self, tr):
+        R = ref_lib()
+        if R is None:
+            raise RuntimeError("oracle/_ref/libteufel_ref.so not built")
+        self.R = R
+        self.tr = tr
+        self.h = R.ref_beam_create(tr.n_particles, tr.n_points, *[_ptr(a) for a in tr.arrays()], _ptr(tr.charge))
+
+    def integrate(self, obs, t0, dt, n_bins, traces=None, p_range=None, n_threads=0):
+        obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+        t0 = np.ascontiguousarray(t0, dtype=np.float64).reshape(-1)
+        if traces is None:
+            traces = np.zeros((obs.shape[0], max(n_bins, 0), 6))
+        p0, p1 = p_range if p_range else (0, self.tr.n_particles)
+        err = self.R.ref_beam_integrate(self.h, p0, p1, obs.shape[0], _ptr(obs), _ptr(t0), dt, n_bins,
+                                        _ptr(traces), n_threads)
+        if err:
+            raise IndexError("reference threw IOexception")
+        return traces
+
+    def retarded_field_at(self, p, obs_time, obs):
+        f = np.zeros(6)
+        o = np.ascontiguousarray(obs, dtype=np.float64)
+        self.R.ref_retarded_field_at(self.h, p, obs_time, _ptr(o), _ptr(f))
+        return f
+
+    def close(self):
+        if self.h:
+            self.R.ref_beam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ref_pseudo_integrate(ts, f, t0, dt, n_bins):
+    R = ref_lib()
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if ts.ndim == 1:
+        ts, f = ts[None], f[None]
+    trace = np.zeros((n_bins, 6))
+    err = R.ref_pseudo_integrate(ts.shape[0], ts.shape[1], _ptr(ts), _ptr(f), t0, dt, n_bins, _ptr(trace))
+    if err:
+        raise IndexError("reference threw IOexception")
+    return trace

@@ -1,0 +1,230 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+reference's golden fixtures.
+
+FP64 tolerance: eps_E <= 1e-12 (teufel_amd.metrics.trace_rel_error).  The
+retarded time, bucket index and bucket edges are computed bit-identically to
+the reference, so what remains is amplitude rounding (~1e-15).
+Mixed tolerance: eps_W <= 1e-5 per pixel (energy), the stated budget.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import teufel_amd
+from teufel_amd import TG_FP64, TG_MIXED
+from teufel_amd import workloads as W
+from teufel_amd.metrics import energy_rel_error, trace_rel_error
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_ENERGY_TOL = 1e-5
+
+
+def say(msg):
+    print("\n[case] " + msg, flush=True)
+
+
+def _traj_from(d):
+    return oracle.Trajectories(**{n: d[n] for n in oracle.Trajectories.NAMES}, charge=d["charge"])
+
+
+def test_deposit_kats_on_device(ctx, golden):
+    say("deposit KATs tests/teufel.integrate_field.cpp")
+    g = golden("kat.npz")
+    for i in range(5):
+        t0, dt, n = g["k%d_par" % i]
+        tr = ctx.deposit_only(g["k%d_ts" % i], g["k%d_f" % i], t0, dt, int(n))
+        assert np.allclose(tr[:, 0], g["k%d_expected" % i], atol=1e-6, rtol=0), i
+        assert np.abs(tr - g["k%d_ref" % i]).max() <= 1e-15, i
+
+
+def test_c1_fp64_matches_reference_fixture(ctx, golden):
+    say("c1 FP64 vs reference fixture")
+    g = golden("c1_ref.npz")
+    tr = _traj_from(g)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(g["obs"], g["t0"], float(g["dt"]), int(g["n_bins"]))
+    ctx.integrate(TG_FP64)
+    out = ctx.read_traces(sid)
+    err = trace_rel_error(out[g["pick"]], g["traces"])
+    print("eps_E(fixture) = %.3e" % err)
+    assert err <= FP64_TOL
+    S = ctx.screen_poynting(sid)
+    assert np.abs(S - g["poynting"]).max() <= 1e-12 * np.abs(g["poynting"]).max()
+    ref = oracle.integrate_screen(tr, g["obs"], g["t0"], float(g["dt"]), int(g["n_bins"]))
+    err2 = trace_rel_error(out, ref)
+    print("eps_E(oracle, all 441 px) = %.3e" % err2)
+    assert err2 <= FP64_TOL
+
+
+def test_bunch_fixture_fp64(ctx, golden):
+    say("3-electron bunch FP64 vs reference fixture")
+    g = golden("bunch_ref.npz")
+    ctx.remove_screens()
+    ctx.set_trajectories(_traj_from(g))
+    sid = ctx.add_screen(g["obs"], g["t0"], float(g["dt"]), int(g["n_bins"]))
+    ctx.integrate(TG_FP64)
+    err = trace_rel_error(ctx.read_traces(sid), g["traces"])
+    print("eps_E = %.3e" % err)
+    assert err <= FP64_TOL
+
+
+def test_loop_point_observer(ctx, golden):
+    say("circular loop, 2-bin point observer (tests/teufel.loop.cpp)")
+    g = golden("loop_ref.npz")
+    ctx.remove_screens()
+    ctx.set_trajectories(_traj_from(g))
+    sid = ctx.add_screen(g["obs"], [float(g["t0"])], float(g["dt"]), int(g["n_bins"]))
+    ctx.integrate(TG_FP64)
+    out = ctx.read_traces(sid)
+    assert trace_rel_error(out, g["traces"]) <= FP64_TOL
+    B = np.linalg.norm(out[0, 0, 3:])
+    assert abs(B / float(g["b_expected"]) - 1.0) < 1e-3
+
+
+def _small_bunch(n_particles=48, n_steps=4000, npx=16, n_bins=256, seed=77):
+    w = W.make("c2", n_particles=n_particles, n_steps=n_steps, seed=seed)
+    u = w.und
+    tr = oracle.track_undulator(w.bunch.x0, w.bunch.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods,
+                                u.origin, charge=-1.0)
+    obs, nrm, area = W.grid_screen(npx, 0.2, 10.0)
+    t_c, x_c = W._centre(u, 60.0)
+    t0, dt = W.set_windows(obs, n_bins, u, 60.0, t_c, x_c, 0.3e-3)
+    return tr, obs, t0, dt, nrm
+
+
+@pytest.mark.parametrize("window,chunk", [(8, 0), (4, 7), (16, 1000), (32, 16)])
+def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk):
+    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d" % (window, chunk))
+    tr, obs, t0, dt, _ = _small_bunch()
+    ref = oracle.integrate_screen(tr, obs, t0, dt, 256)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 256)
+    ctx.integrate(TG_FP64, tuning={"bins_per_window": window, "particles_per_chunk": chunk})
+    err = trace_rel_error(ctx.read_traces(sid), ref)
+    print("eps_E = %.3e" % err)
+    assert err <= FP64_TOL
+
+
+def test_mixed_energy_error_and_determinism(ctx):
+    say("48 e- mixed vs FP64, energy per pixel; repeat bitwise")
+    tr, obs, t0, dt, nrm = _small_bunch()
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 256)
+    ctx.integrate(TG_FP64)
+    S64 = ctx.screen_poynting(sid)
+    T64 = ctx.read_traces(sid)
+    ctx.zero_screen(sid)
+    ctx.integrate(TG_MIXED)
+    Tm = ctx.read_traces(sid)
+    Sm = ctx.screen_poynting(sid)
+    ctx.zero_screen(sid)
+    ctx.integrate(TG_MIXED)
+    Tm2 = ctx.read_traces(sid)
+    assert np.array_equal(Tm, Tm2)
+    epsW, excl = energy_rel_error(W.energy_density(Sm, nrm), W.energy_density(S64, nrm))
+    epsE = trace_rel_error(Tm, T64)
+    print("mixed: eps_W = %.3e (excluded %.3f), eps_E = %.3e" % (epsW, excl, epsE))
+    assert epsW <= MIXED_ENERGY_TOL
+
+
+def test_gpu_pusher_matches_oracle(ctx):
+    say("device Vay pusher vs oracle pusher, 8 e- x 4000 steps")
+    w = W.make("c2", n_particles=8, n_steps=4000, seed=5)
+    u = w.und
+    ctx.track_undulator(w.bunch.x0, w.bunch.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods, u.origin)
+    d = ctx.get_trajectories()
+    tr = oracle.track_undulator(w.bunch.x0, w.bunch.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods,
+                                u.origin)
+    assert np.array_equal(d["t"], tr.t)
+    for n in ("x", "y", "z"):
+        assert np.abs(d[n] - getattr(tr, n)).max() <= 1e-12
+    for n in ("ux", "uy", "uz"):
+        assert np.abs(d[n] - getattr(tr, n)).max() <= 1e-10 * 60
+    amax = np.abs(tr.ax).max()
+    for n in ("ax", "ay", "az"):
+        assert np.abs(d[n] - getattr(tr, n)).max() <= 1e-10 * amax
+
+
+def test_batched_screens_equal_single(ctx):
+    say("3 uneven screens batched vs one at a time, FP64, bitwise")
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=24, n_steps=2000)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    geo = [(obs, t0, dt, 256)]
+    o2, _, _ = W.grid_screen(7, 0.05, 5.0, tilt_deg=20.0)
+    u = W.Undulator()
+    t_c, x_c = W._centre(u, 60.0)
+    t02, dt2 = W.set_windows(o2, 100, u, 60.0, t_c, x_c, 0.3e-3)
+    geo.append((o2, t02, dt2, 100))
+    o3, _, _ = W.grid_screen(9, 0.1, 3.0, toroid_f=2.0)
+    t03, dt3 = W.set_windows(o3, 513, u, 60.0, t_c, x_c, 0.3e-3)
+    geo.append((o3, t03, dt3, 513))
+    ids = [ctx.add_screen(*g) for g in geo]
+    ctx.integrate(TG_FP64, tuning={"bins_per_window": 8, "particles_per_chunk": 8})
+    batched = [ctx.read_traces(i) for i in ids]
+    for i, g in zip(ids, geo):
+        ctx.zero_screen(i)
+    singles = []
+    for i in ids:
+        ctx.integrate_range(TG_FP64, 0, tr.n_particles, screen_id=i, tuning={"bins_per_window": 8, "particles_per_chunk": 8})
+        singles.append(ctx.read_traces(i))
+    for a, b_, g in zip(batched, singles, geo):
+        assert np.array_equal(a, b_)
+        ref = oracle.integrate_screen(tr, g[0], g[1], g[2], g[3])
+        assert trace_rel_error(a, ref) <= FP64_TOL
+
+
+def test_graph_replay_matches_eager(ctx):
+    say("hipGraph replays vs eager, mixed")
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=16, n_steps=2000)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 256)
+    ctx.integrate(TG_MIXED)
+    eager = ctx.read_traces(sid)
+    for rep in range(4):
+        ctx.zero_screen(sid)
+        ctx.integrate(TG_MIXED, tuning={"use_graph": 1})
+        out = ctx.read_traces(sid)
+        np.abs(out).sum()  # host work between replays
+        assert np.array_equal(out, eager), rep
+
+
+def test_accumulate_and_noops(ctx):
+    say("accumulate semantics, n_bins<2 and NP<2 no-ops")
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=8, n_steps=1000)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 64)
+    s1 = ctx.add_screen(obs[:5], t0[:5], dt, 1)
+    ctx.integrate(TG_FP64)
+    a = ctx.read_traces(sid)
+    ctx.integrate(TG_FP64)
+    b = ctx.read_traces(sid)
+    assert np.allclose(b, 2 * a, rtol=1e-14, atol=0)
+    host = np.ones_like(a)
+    ctx.read_traces(sid, out=host, accumulate=True)
+    assert np.array_equal(host, 1.0 + b)
+    assert ctx.read_traces(s1).shape == (5, 1, 6)
+    one = oracle.Trajectories.empty(3, 1)
+    ctx.set_trajectories(one)
+    ctx.zero_screen(sid)
+    ctx.integrate(TG_FP64)
+    assert not ctx.read_traces(sid).any()
+
+
+def test_errors_are_reported(ctx):
+    say("argument errors map to TgError")
+    with pytest.raises(teufel_amd.TgError):
+        ctx.add_screen(np.zeros((2, 3)), np.zeros(2), -1.0, 10)
+    with pytest.raises(teufel_amd.TgError):
+        ctx.add_screen(np.array([[0, 0, math.nan]]), np.zeros(1), 1e-15, 10)
+    with pytest.raises(teufel_amd.TgError):
+        ctx._chk(ctx.lib.tg_zero_screen(ctx.h, 9999))

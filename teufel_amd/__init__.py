@@ -1,0 +1,187 @@
+"""teufel_amd - MI355X (gfx950) Lienard-Wiechert radiation step of TEUFEL.
+
+Python access to the C ABI (include/teufel_gpu.h) for tests and benchmarks.
+The C++ host mirror of the reference API (ChargedParticle, Bunch, Beam,
+MeshedScreen, ScreenObserver, PointObserver) lives in teufel_amd/host/.
+
+Every compute call goes to the HIP library; if it is not built, or no GPU is
+present, the call raises - there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+from ._native import TG_FP64, TG_MIXED, TgTuning
+
+__all__ = ["TG_FP64", "TG_MIXED", "TgTuning", "TgError", "RadiationContext", "load_library"]
+
+_ERRNAMES = {1: "ARG", 2: "OOM", 3: "HIP", 4: "RANGE", 5: "STATE"}
+
+
+class TgError(IOError):
+    """Non-zero status of a tg_* call (the reference raises IOexception)."""
+
+    def __init__(self, code, msg):
+        super().__init__("TG_ERR_%s (%d): %s" % (_ERRNAMES.get(code, "?"), code, msg))
+        self.code = code
+
+
+def load_library():
+    return _native.load()
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class RadiationContext:
+    """One device context: trajectories, screens and their on-device traces."""
+
+    def __init__(self, device=0):
+        self.lib = _native.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.tg_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise TgError(rc, "tg_create(%d) failed (is a GPU present?)" % device)
+        self.h = h
+        self.screens = []  # (n_pix, n_bins)
+
+    # ---- plumbing ----
+    def _chk(self, rc):
+        if rc != 0:
+            raise TgError(rc, self.lib.tg_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tg_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        buf = ctypes.create_string_buffer(256)
+        n = ctypes.c_int()
+        self._chk(self.lib.tg_device_info(self.h, buf, 256, ctypes.byref(n)))
+        return buf.value.decode(), n.value
+
+    # ---- trajectories ----
+    def set_trajectories(self, tr):
+        """tr: object with t,x,y,z,ux,uy,uz,ax,ay,az (n_particles, n_points) and charge."""
+        arrs = [_f64(getattr(tr, n)) for n in ("t", "x", "y", "z", "ux", "uy", "uz", "ax", "ay", "az")]
+        n_p, n_pts = arrs[0].shape if arrs[0].ndim == 2 else (1, arrs[0].shape[0])
+        q = _f64(tr.charge).reshape(-1)
+        self._keep = arrs + [q]
+        self._chk(self.lib.tg_set_trajectories(self.h, n_p, n_pts, *[_dptr(a) for a in arrs], _dptr(q)))
+
+    def track_undulator(self, x0, u0, dt, n_steps, bpeak, lambda_u, n_periods, origin,
+                        charge=-1.0, mass=1.0, t_start=0.0):
+        x0 = _f64(x0, (-1, 3))
+        u0 = _f64(u0, (-1, 3))
+        o = _f64(origin, (3,))
+        self._chk(self.lib.tg_track_undulator(self.h, x0.shape[0], _dptr(x0), _dptr(u0), t_start, charge,
+                                              mass, dt, n_steps, bpeak, lambda_u, n_periods, _dptr(o)))
+
+    def trajectory_shape(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.lib.tg_trajectory_shape(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def get_trajectories(self, p_begin=0, p_end=None):
+        n_p, n_pts = self.trajectory_shape()
+        p_end = n_p if p_end is None else p_end
+        out = {n: np.zeros((p_end - p_begin, n_pts)) for n in
+               ("t", "x", "y", "z", "ux", "uy", "uz", "ax", "ay", "az")}
+        out["charge"] = np.zeros(p_end - p_begin)
+        names = ("t", "x", "y", "z", "ux", "uy", "uz", "ax", "ay", "az", "charge")
+        self._chk(self.lib.tg_get_trajectories(self.h, p_begin, p_end, *[_dptr(out[n]) for n in names]))
+        return out
+
+    # ---- screens ----
+    def add_screen(self, obs, t0, dt, n_bins):
+        obs = _f64(obs, (-1, 3))
+        t0 = _f64(t0, (-1,))
+        sid = ctypes.c_int()
+        self._chk(self.lib.tg_add_screen(self.h, obs.shape[0], _dptr(obs), _dptr(t0), float(dt), int(n_bins),
+                                         ctypes.byref(sid)))
+        self.screens.append((obs.shape[0], int(n_bins)))
+        return sid.value
+
+    def zero_screen(self, sid):
+        self._chk(self.lib.tg_zero_screen(self.h, sid))
+
+    def remove_screens(self):
+        self._chk(self.lib.tg_remove_screens(self.h))
+        self.screens = []
+
+    # ---- integration ----
+    @staticmethod
+    def _tuning(tuning):
+        if tuning is None:
+            return None
+        if isinstance(tuning, TgTuning):
+            return ctypes.byref(tuning)
+        t = TgTuning(**tuning)
+        return ctypes.byref(t)
+
+    def integrate(self, precision=TG_FP64, tuning=None):
+        self._chk(self.lib.tg_integrate(self.h, precision, self._tuning(tuning)))
+
+    def integrate_range(self, precision, p_begin, p_end, screen_id=-1, tuning=None):
+        self._chk(self.lib.tg_integrate_range(self.h, precision, self._tuning(tuning), screen_id, p_begin, p_end))
+
+    def synchronize(self):
+        self._chk(self.lib.tg_synchronize(self.h))
+
+    def last_timing(self):
+        k, t = ctypes.c_float(), ctypes.c_float()
+        self._chk(self.lib.tg_last_timing(self.h, ctypes.byref(k), ctypes.byref(t)))
+        return k.value, t.value
+
+    def autotune(self, precision=TG_MIXED):
+        t = TgTuning()
+        self._chk(self.lib.tg_autotune(self.h, precision, ctypes.byref(t)))
+        return {"bins_per_window": t.bins_per_window, "particles_per_chunk": t.particles_per_chunk,
+                "use_graph": t.use_graph}
+
+    def read_traces(self, sid, out=None, accumulate=False):
+        n_pix, n_bins = self.screens[sid]
+        if out is None:
+            out = np.zeros((n_pix, n_bins, 6))
+            accumulate = False
+        self._chk(self.lib.tg_read_traces(self.h, sid, _dptr(out), 1 if accumulate else 0))
+        return out
+
+    def screen_poynting(self, sid):
+        n_pix, _ = self.screens[sid]
+        out = np.zeros((n_pix, 3))
+        self._chk(self.lib.tg_screen_poynting(self.h, sid, _dptr(out)))
+        return out
+
+    def deposit_only(self, ts, fields, t0, dt, n_bins, trace=None):
+        ts = _f64(ts)
+        fields = _f64(fields)
+        if ts.ndim == 1:
+            ts, fields = ts[None], fields[None]
+        if trace is None:
+            trace = np.zeros((max(int(n_bins), 0), 6))
+        self._chk(self.lib.tg_deposit_only(self.h, ts.shape[0], ts.shape[1], _dptr(ts), _dptr(fields),
+                                           float(t0), float(dt), int(n_bins), _dptr(trace)))
+        return trace

@@ -1,0 +1,477 @@
+// lw_kernels.hip - gfx950 kernels of the post-tracking radiation step.
+//
+// Reference path: ChargedParticle::integrateFieldTrace (src/particle.cpp:771-867)
+// called per pixel by the OpenMP observer loops (src/mesh.cpp:288-342,
+// src/screen.cpp:112-180) and per particle by Bunch/Beam
+// (src/bunch.cpp:739-745, src/beam.cpp:443-450).
+//
+// Decomposition.  One wavefront = 64 pixels (one per lane, a "tile") x one
+// chunk of particles.  The tile's traces are produced window by window: a
+// window of W bins x 6 components x 64 lanes lives in LDS (FP64).  For every
+// particle of the chunk the wave walks the trajectory steps whose retarded-time
+// interval touches the window, evaluating each trajectory point once (the
+// reference evaluates each point twice) and depositing with the reference's
+// 3-case bucket rule.  A lane only ever writes its own pixel's LDS cells, in
+// particle order and step order, so the deposit needs no atomics and is
+// bitwise reproducible.  Per (work item, particle) a cursor remembers the
+// first step that reaches the next window, so every step is walked about
+// once per work item.  Trajectory points are uniform across the wave and are
+// read with scalar loads.  Chunks write partial traces that a second kernel
+// sums in fixed chunk order into the screen's trace store.
+#include "tg_device.h"
+
+#include <climits>
+
+namespace tg {
+
+// ---------------------------------------------------------------------------
+// Packing: raw ChargedParticle trajectory (t, X, u=beta*gamma, a) -> records.
+// ---------------------------------------------------------------------------
+
+__global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict__ t,
+                                const double* __restrict__ x, const double* __restrict__ y,
+                                const double* __restrict__ z, const double* __restrict__ ux,
+                                const double* __restrict__ uy, const double* __restrict__ uz,
+                                const double* __restrict__ ax, const double* __restrict__ ay,
+                                const double* __restrict__ az, const double* __restrict__ charge,
+                                RecF64* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    size_t p = g / (size_t)n_points;
+    // src/particle.cpp:575-582: gamma2 = |u|^2 + 1, beta = u/gamma, beta' = a/gamma
+    double bg2 = ux[g] * ux[g] + uy[g] * uy[g] + uz[g] * uz[g];
+    double gamma2 = bg2 + 1.0;
+    double gamma = __builtin_sqrt(gamma2);
+    RecF64 r;
+    r.t = t[g];
+    r.x = x[g];
+    r.y = y[g];
+    r.z = z[g];
+    r.bx = ux[g] / gamma;
+    r.by = uy[g] / gamma;
+    r.bz = uz[g] / gamma;
+    r.bpx = ax[g] / gamma;
+    r.bpy = ay[g] / gamma;
+    r.bpz = az[g] / gamma;
+    r.g2inv = 1.0 / gamma2;
+    r.scale = charge[p] * kElementaryCharge / (4.0 * kPi * kEpsNull);  // particle.cpp:588
+    out[g] = r;
+}
+
+__global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restrict__ t,
+                                  const double* __restrict__ x, const double* __restrict__ y,
+                                  const double* __restrict__ z, const double* __restrict__ ux,
+                                  const double* __restrict__ uy, const double* __restrict__ uz,
+                                  const double* __restrict__ ax, const double* __restrict__ ay,
+                                  const double* __restrict__ az, const double* __restrict__ charge,
+                                  RecMixed* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    size_t p = g / (size_t)n_points;
+    double bg2 = ux[g] * ux[g] + uy[g] * uy[g] + uz[g] * uz[g];
+    double gamma2 = bg2 + 1.0;
+    double gamma = __builtin_sqrt(gamma2);
+    double scale = charge[p] * kElementaryCharge / (4.0 * kPi * kEpsNull);
+    RecMixed r;
+    r.t = t[g];
+    r.x = x[g];
+    r.y = y[g];
+    r.z = z[g];
+    r.bx = ux[g] / gamma;
+    r.by = uy[g] / gamma;
+    r.bz = uz[g] / gamma;
+    r.bpx = (float)(ax[g] / gamma);
+    r.bpy = (float)(ay[g] / gamma);
+    r.bpz = (float)(az[g] / gamma);
+    r.sv = (float)(scale / gamma2);
+    r.sa = (float)(scale / kSpeedOfLight);
+    r.fbx = (float)r.bx;
+    r.fby = (float)r.by;
+    r.fbz = (float)r.bz;
+    out[g] = r;
+}
+
+// ---------------------------------------------------------------------------
+// One trajectory point seen from one pixel.
+// ---------------------------------------------------------------------------
+
+struct Point {
+    double ts;     // retarded (arrival) time, bit-identical to the reference
+    int idx;       // round((ts - t0)/dt), bit-identical to the reference
+    double f[6];   // Ex, Ey, Ez, Bx, By, Bz
+};
+
+// Retarded time t + |P - X|/c and bucket index: src/particle.cpp:560-568,
+// 805, 829-830, in the reference's operation order.
+__device__ __forceinline__ void timing_exact(double t, double dx, double dy, double dz, double t0,
+                                             double dt, double inv_dt, double& R, double& ts, int& idx)
+{
+#pragma clang fp contract(off)
+    double r2 = dx * dx + dy * dy + dz * dz;
+    R = __builtin_sqrt(r2);
+    ts = t + div_by_const(R, kSpeedOfLight, kInvSpeedOfLight);
+    idx = round_index(div_by_const(ts - t0, dt, inv_dt));
+}
+
+// FP64 Lienard-Wiechert field (src/particle.cpp:570-596).  n = RVec/R is
+// formed bit-identically (Markstein with RN(1/R)) and 1 - n.beta in the
+// reference's summation order, because that difference cancels ~4 digits.
+// The acceleration term uses a x (b x c) = b (a.c) - c (a.b).
+__device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, double Pz, double t0,
+                                         double dt, double inv_dt, Point& o)
+{
+    double dx, dy, dz, R, nx, ny, nz, kappa, invR;
+    {
+#pragma clang fp contract(off)
+        dx = Px - r.x;
+        dy = Py - r.y;
+        dz = Pz - r.z;
+        timing_exact(r.t, dx, dy, dz, t0, dt, inv_dt, R, o.ts, o.idx);
+        invR = 1.0 / R;
+        nx = div_by_const(dx, R, invR);
+        ny = div_by_const(dy, R, invR);
+        nz = div_by_const(dz, R, invR);
+        double nb = r.bx * nx + r.by * ny + r.bz * nz;
+        kappa = 1.0 - nb;
+    }
+    double mx = nx - r.bx, my = ny - r.by, mz = nz - r.bz;   // n - beta
+    double k3 = kappa * kappa * kappa;
+    double irk3 = 1.0 / (R * k3);                              // 1/(R kappa^3)
+    double cv = r.g2inv * invR * irk3;                        // 1/(R^2 kappa^3 gamma^2)
+    double ca = irk3 * kInvSpeedOfLight;                      // 1/(R kappa^3 c)
+    double nbp = nx * r.bpx + ny * r.bpy + nz * r.bpz;        // n . beta'
+    double nm = nx * mx + ny * my + nz * mz;                  // n . (n - beta)
+    double c1 = (cv + ca * nbp) * r.scale;
+    double c2 = ca * nm * r.scale;
+    double ex = mx * c1 - r.bpx * c2;
+    double ey = my * c1 - r.bpy * c2;
+    double ez = mz * c1 - r.bpz * c2;
+    o.f[0] = ex;
+    o.f[1] = ey;
+    o.f[2] = ez;
+    o.f[3] = (ny * ez - nz * ey) * kInvSpeedOfLight;           // B = n x E / c
+    o.f[4] = (nz * ex - nx * ez) * kInvSpeedOfLight;
+    o.f[5] = (nx * ey - ny * ex) * kInvSpeedOfLight;
+}
+
+// Mixed precision: FP64 distance, retarded time and 1 - n.beta (the two
+// quantities whose cancellation FP32 cannot carry), FP32 amplitudes.
+__device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double Py, double Pz,
+                                           double t0, double dt, double inv_dt, Point& o)
+{
+    double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    // 1/sqrt: FP32 hardware estimate + one FP64 Newton step (~2^-46 relative)
+    double y = (double)__builtin_amdgcn_rsqf((float)r2);
+    double h = r2 * y;
+    double e = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+    double R = r2 * y;
+    o.ts = __builtin_fma(R, kInvSpeedOfLight, r.t);
+    o.idx = round_index((o.ts - t0) * inv_dt);
+    double nx = dx * y, ny = dy * y, nz = dz * y;
+    double kappa = 1.0 - (r.bx * nx + r.by * ny + r.bz * nz);
+    float fnx = (float)nx, fny = (float)ny, fnz = (float)nz;
+    float mx = (float)(nx - r.bx), my = (float)(ny - r.by), mz = (float)(nz - r.bz);
+    float fk = (float)kappa;
+    float fy = (float)y;
+    float irk3 = fy * __builtin_amdgcn_rcpf(fk * fk * fk);
+    float cv = r.sv * fy * irk3;
+    float ca = r.sa * irk3;
+    float nbp = fnx * r.bpx + fny * r.bpy + fnz * r.bpz;
+    float c1 = cv + ca * nbp;
+    float c2 = ca * fk;                                       // n.(n-beta) = kappa for unit n
+    float ex = mx * c1 - r.bpx * c2;
+    float ey = my * c1 - r.bpy * c2;
+    float ez = mz * c1 - r.bpz * c2;
+    const float ic = (float)kInvSpeedOfLight;
+    o.f[0] = ex;
+    o.f[1] = ey;
+    o.f[2] = ez;
+    o.f[3] = (fny * ez - fnz * ey) * ic;
+    o.f[4] = (fnz * ex - fnx * ez) * ic;
+    o.f[5] = (fnx * ey - fny * ex) * ic;
+}
+
+template <int PREC>
+struct RecOf;
+template <>
+struct RecOf<0> { using type = RecF64; };
+template <>
+struct RecOf<1> { using type = RecMixed; };
+
+template <int PREC>
+__device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, double Px, double Py,
+                                           double Pz, double t0, double dt, double inv_dt, Point& o)
+{
+    if constexpr (PREC == 0)
+        eval_f64(r, Px, Py, Pz, t0, dt, inv_dt, o);
+    else
+        eval_mixed(r, Px, Py, Pz, t0, dt, inv_dt, o);
+}
+
+// ---------------------------------------------------------------------------
+// The bucket deposit of one trajectory step, src/particle.cpp:788-865,
+// restricted to bins [b0, b1).  Every contribution is alpha*f1 + beta*f2;
+// time arithmetic (bucket edges/centres) is done as the reference does it.
+// acc(j, alpha, beta) adds the contribution into bin j.
+// ---------------------------------------------------------------------------
+template <class Acc>
+__device__ __forceinline__ void deposit_step(const Acc& acc, bool active, double ts1, int idx1,
+                                             double ts2, int idx2, double t0, double dt,
+                                             double inv_dt, double tmin, double tmax, int n_bins,
+                                             int b0, int b1)
+{
+#pragma clang fp contract(off)
+    if (!(active && ts2 > tmin && ts1 < tmax)) return;
+    const bool caseA = ts1 <= tmin;
+    const int i2 = idx2 > n_bins ? n_bins : idx2;
+    const int lo = caseA ? 0 : idx1;
+    const int hi = i2 < n_bins - 1 ? i2 : n_bins - 1;
+    const int jlo = lo > b0 ? lo : b0;
+    const int jhi = hi < b1 - 1 ? hi : b1 - 1;
+    if (jlo > jhi) return;
+    const double dts = ts2 - ts1;
+    if (!caseA && idx1 == i2) {
+        // step within one bucket: (f1+f2)*0.5*(dts/dt)
+        double w = 0.5 * (dts * inv_dt);
+        acc(idx1, w, w);
+        return;
+    }
+    const double idts = 1.0 / dts;
+    for (int j = jlo; j <= jhi; ++j) {
+        double a, b;
+        if (!caseA && j == idx1) {
+            // first, partially covered bucket
+            double t_end = t0 + (idx1 + 0.5) * dt;
+            double ae = (ts2 - t_end) * idts;
+            double be = (t_end - ts1) * idts;
+            double h = 0.5 * ((t_end - ts1) * inv_dt);
+            a = h * (1.0 + ae);
+            b = h * be;
+        } else if (j == i2) {
+            // last, partially covered bucket (i2 < n_bins here)
+            double t_start = t0 + (i2 - 0.5) * dt;
+            double as = (ts2 - t_start) * idts;
+            double bs = (t_start - ts1) * idts;
+            double h = 0.5 * ((ts2 - t_start) * inv_dt);
+            a = h * as;
+            b = h * (bs + 1.0);
+        } else {
+            // fully covered bucket: value at the bucket centre
+            double tc = t0 + j * dt;
+            a = (ts2 - tc) * idts;
+            b = (tc - ts1) * idts;
+        }
+        acc(j, a, b);
+    }
+}
+
+// LDS window accumulator: cell (bin j, component c, lane) at ((j-b0)*6+c)*64+lane.
+struct LdsAcc {
+    double* base;  // lds + lane
+    int b0;
+    const double* f1;
+    const double* f2;
+    __device__ __forceinline__ void operator()(int j, double a, double b) const
+    {
+        double* cell = base + (j - b0) * (6 * kLanes);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cell[c * kLanes] += a * f1[c] + b * f2[c];
+    }
+};
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Walk the steps of one particle starting at step k (uniform) for the window
+// [b0, b1).  Returns (through nk/nmin) the first step that reaches bins >= b1
+// for some lane and the smallest start bin of that step over the lanes.
+template <int PREC>
+__device__ __forceinline__ void walk_particle(const typename RecOf<PREC>::type* __restrict__ rp, int k,
+                                              int NP, bool active, double Px, double Py, double Pz,
+                                              double t0, double dt, double inv_dt, double tmin,
+                                              double tmax, int n_bins, int b0, int b1,
+                                              double* lds_lane, int& nk, int& nmin)
+{
+    Point p1, p2;
+    eval_point<PREC>(rp[k], Px, Py, Pz, t0, dt, inv_dt, p1);
+    int next_k = -1, next_min = INT_MAX;
+    while (true) {
+        if (!__ballot(active && p1.idx < b1)) {
+            if (next_k < 0) {
+                next_k = k;
+                next_min = wave_min(active ? p1.idx : INT_MAX);
+            }
+            break;
+        }
+        eval_point<PREC>(rp[k + 1], Px, Py, Pz, t0, dt, inv_dt, p2);
+        LdsAcc acc{lds_lane, b0, p1.f, p2.f};
+        deposit_step(acc, active, p1.ts, p1.idx, p2.ts, p2.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        if (next_k < 0 && __ballot(active && p2.idx >= b1)) {
+            next_k = k;
+            next_min = wave_min(active ? p1.idx : INT_MAX);
+        }
+        ++k;
+        if (k >= NP - 1) {
+            if (next_k < 0) {
+                next_k = NP - 1;
+                next_min = INT_MAX;
+            }
+            break;
+        }
+        p1 = p2;
+    }
+    nk = next_k;
+    nmin = next_min;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items,
+                                                        const ScreenDev* __restrict__ screens,
+                                                        const typename RecOf<PREC>::type* __restrict__ recs,
+                                                        int n_points, int2* __restrict__ cursors)
+{
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const WorkItem it = items[blockIdx.x];
+    const ScreenDev S = screens[it.screen];
+    const int pix = S.perm[it.tile * kLanes + lane];
+    const bool active = pix >= 0;
+    const int pl = active ? pix : S.perm[it.tile * kLanes];  // lane 0 of a tile is a real pixel
+    const double Px = S.obs[3 * pl], Py = S.obs[3 * pl + 1], Pz = S.obs[3 * pl + 2];
+    const double t0 = S.t0[pl];
+    const double dt = S.dt, inv_dt = S.inv_dt;
+    const int nb = S.n_bins, W = S.window, NP = n_points;
+    double tmin, tmax;
+    {
+#pragma clang fp contract(off)
+        tmin = t0 - dt * 0.5;                         // particle.cpp:780-781
+        tmax = t0 + dt * ((double)nb - 0.5);
+    }
+    int2* __restrict__ cur_base = cursors + it.cursor_off;
+    double* lds_lane = lds + lane;
+    for (int b0 = 0; b0 < nb; b0 += W) {
+        const int b1 = b0 + W < nb ? b0 + W : nb;
+        const int ncell = (b1 - b0) * 6;
+        for (int j = 0; j < ncell; ++j) lds_lane[j * kLanes] = 0.0;
+        for (int pb = it.p0; pb < it.p1; pb += kLanes) {
+            const int npb = it.p1 - pb < kLanes ? it.p1 - pb : kLanes;
+            int2 cur = make_int2(0, INT_MIN);
+            if (b0 > 0 && lane < npb) cur = cur_base[pb - it.p0 + lane];
+            unsigned long long todo = __ballot(lane < npb && cur.x < NP - 1 && cur.y < b1);
+            int2 ncur = cur;
+            while (todo) {
+                const int j = __ffsll((long long)todo) - 1;
+                todo &= todo - 1;
+                const int k = __builtin_amdgcn_readfirstlane(__shfl(cur.x, j));
+                const int p = pb + j;
+                int nk, nmin;
+                walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
+                                    tmin, tmax, nb, b0, b1, lds_lane, nk, nmin);
+                if (lane == j) ncur = make_int2(nk, nmin);
+            }
+            if (b1 < nb && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
+        }
+        if (active) {
+            double* out = S.partial + (((size_t)it.chunk * S.n_pix + pix) * nb + b0) * 6;
+            for (int j = 0; j < ncell; ++j) out[j] = lds_lane[j * kLanes];
+        }
+    }
+}
+
+template __global__ void radiation_kernel<0>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*);
+template __global__ void radiation_kernel<1>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*);
+
+// ---------------------------------------------------------------------------
+// Sum the chunk partials into the trace store, chunks in order.
+// ---------------------------------------------------------------------------
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, double* __restrict__ out,
+                                       size_t n, int n_chunks)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double s = out[i];
+        for (int c = 0; c < n_chunks; ++c) s += partial[(size_t)c * n + i];
+        out[i] = s;
+    }
+}
+
+__global__ void fill_kernel(double* __restrict__ p, size_t n, double v)
+{
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Deposit only: injected (t_ret, field) per point of pseudo-particles into one
+// trace in global memory, with the deposit code of the radiation kernel.
+// ---------------------------------------------------------------------------
+struct GlobalAcc {
+    double* trace;
+    const double* f1;
+    const double* f2;
+    __device__ __forceinline__ void operator()(int j, double a, double b) const
+    {
+        double* cell = trace + 6 * j;
+        for (int c = 0; c < 6; ++c) cell[c] += a * f1[c] + b * f2[c];
+    }
+};
+
+__global__ void deposit_only_kernel(int n_pseudo, int n_points, const double* __restrict__ ts,
+                                    const double* __restrict__ f, double t0, double dt, double inv_dt,
+                                    int n_bins, double* __restrict__ trace)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double tmin, tmax;
+    {
+#pragma clang fp contract(off)
+        tmin = t0 - dt * 0.5;
+        tmax = t0 + dt * ((double)n_bins - 0.5);
+    }
+    for (int p = 0; p < n_pseudo; ++p) {
+        const double* tp = ts + (size_t)p * n_points;
+        const double* fp = f + (size_t)p * n_points * 6;
+        for (int k = 0; k + 1 < n_points; ++k) {
+            double ts1 = tp[k], ts2 = tp[k + 1];
+            int i1 = round_index(div_by_const(ts1 - t0, dt, inv_dt));
+            int i2 = round_index(div_by_const(ts2 - t0, dt, inv_dt));
+            GlobalAcc acc{trace, fp + 6 * k, fp + 6 * (k + 1)};
+            deposit_step(acc, true, ts1, i1, ts2, i2, t0, dt, inv_dt, tmin, tmax, n_bins, 0, n_bins);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel Poynting integral S = dt * sum E x B / mu0 (src/fieldtrace.cpp:192-198).
+// ---------------------------------------------------------------------------
+__global__ void poynting_kernel(const double* __restrict__ traces, int n_pix, int n_bins, double dt,
+                                double* __restrict__ out)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pix) return;
+    const double* tr = traces + (size_t)p * n_bins * 6;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int i = 0; i < n_bins; ++i) {
+        const double* f = tr + 6 * i;
+        sx += (f[1] * f[5] - f[2] * f[4]) / kMuNull;
+        sy += (f[2] * f[3] - f[0] * f[5]) / kMuNull;
+        sz += (f[0] * f[4] - f[1] * f[3]) / kMuNull;
+    }
+    out[3 * p] = sx * dt;
+    out[3 * p + 1] = sy * dt;
+    out[3 * p + 2] = sz * dt;
+}
+
+}  // namespace tg

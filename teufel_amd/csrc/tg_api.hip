@@ -1,0 +1,880 @@
+// tg_api.hip - host side of the C ABI declared in include/teufel_gpu.h.
+//
+// One tg_context = one device, one stream, one trajectory set and a list of
+// screens whose traces live in HBM (the on-device trace store that replaces
+// the per-rank getBuffer / MPI_Reduce / fromBuffer of
+// src/teufel_hybrid.cpp:726-763).  Kernels are in lw_kernels.hip and
+// vay_kernel.hip, compiled into this translation unit.
+#include "lw_kernels.hip"
+#include "vay_kernel.hip"
+
+#include "../../include/teufel_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace tg;
+
+namespace {
+
+constexpr int kNumRaw = 10;  // t, x, y, z, ux, uy, uz, ax, ay, az
+
+struct ScreenHost {
+    int n_pix = 0, n_bins = 0;
+    double dt = 0.0;
+    std::vector<double> obs, t0;
+    double* d_obs = nullptr;
+    double* d_t0 = nullptr;
+    double* d_traces = nullptr;
+    int* d_perm = nullptr;
+    int n_tiles = 0;
+    bool perm_valid = false;
+    int window = 0;  // autotuned bins per window (0 = default)
+};
+
+struct LaunchPlan {
+    int n_chunks = 0;
+    int chunk = 0;  // particles per chunk
+    std::vector<WorkItem> items;
+    std::vector<ScreenDev> sdev;
+    std::vector<int> screen_ids;
+    size_t partial_doubles = 0;
+    size_t cursor_count = 0;
+    int max_window = 0;
+};
+
+}  // namespace
+
+struct tg_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    char name[256] = {0};
+    int n_cu = 0;
+    size_t max_lds = 65536;
+    // trajectories
+    int n_particles = 0, n_points = 0;
+    double* d_raw = nullptr;     // kNumRaw arrays of n_particles*n_points, then charge[n_particles]
+    void* d_rec[2] = {nullptr, nullptr};
+    bool rec_valid[2] = {false, false};
+    // screens
+    std::vector<ScreenHost> screens;
+    // launch scratch
+    WorkItem* d_items = nullptr;
+    size_t cap_items = 0;
+    ScreenDev* d_sdev = nullptr;
+    size_t cap_sdev = 0;
+    int2* d_cursors = nullptr;
+    size_t cap_cursors = 0;
+    double* d_partial = nullptr;
+    size_t cap_partial = 0;
+    // timing
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float kernel_ms = 0.f, total_ms = 0.f;
+    // tuning
+    tg_tuning tuned[2] = {};
+    bool has_tuned[2] = {false, false};
+    // graph cache
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::string graph_key;
+};
+
+namespace {
+
+int fail(tg_context* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define TG_HIP(c, call)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((c), e_ == hipErrorOutOfMemory ? TG_ERR_OOM : TG_ERR_HIP,             \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+template <class T>
+int ensure_cap(tg_context* c, T*& ptr, size_t& cap, size_t n)
+{
+    if (n <= cap && ptr) return TG_OK;
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        (void)e;
+        ptr = nullptr;
+        cap = 0;
+    }
+    size_t want = std::max<size_t>(n, 1);
+    TG_HIP(c, hipMalloc((void**)&ptr, want * sizeof(T)));
+    cap = want;
+    return TG_OK;
+}
+
+void free_graph(tg_context* c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_key.clear();
+}
+
+void invalidate_traj(tg_context* c)
+{
+    c->rec_valid[0] = c->rec_valid[1] = false;
+    for (auto& s : c->screens) s.perm_valid = false;
+    free_graph(c);
+}
+
+int free_traj(tg_context* c)
+{
+    if (c->d_raw) TG_HIP(c, hipFree(c->d_raw));
+    for (int i = 0; i < 2; ++i)
+        if (c->d_rec[i]) TG_HIP(c, hipFree(c->d_rec[i]));
+    c->d_raw = nullptr;
+    c->d_rec[0] = c->d_rec[1] = nullptr;
+    c->n_particles = c->n_points = 0;
+    invalidate_traj(c);
+    return TG_OK;
+}
+
+double* raw(tg_context* c, int i) { return c->d_raw + (size_t)i * c->n_particles * c->n_points; }
+double* raw_charge(tg_context* c) { return c->d_raw + (size_t)kNumRaw * c->n_particles * c->n_points; }
+
+int alloc_traj(tg_context* c, int n_particles, int n_points)
+{
+    int rc = free_traj(c);
+    if (rc) return rc;
+    size_t n = (size_t)n_particles * (size_t)n_points;
+    TG_HIP(c, hipMalloc((void**)&c->d_raw, (kNumRaw * n + (size_t)n_particles) * sizeof(double)));
+    c->n_particles = n_particles;
+    c->n_points = n_points;
+    return TG_OK;
+}
+
+int ensure_records(tg_context* c, int prec)
+{
+    if (c->rec_valid[prec]) return TG_OK;
+    size_t n = (size_t)c->n_particles * c->n_points;
+    if (!c->d_rec[prec]) TG_HIP(c, hipMalloc(&c->d_rec[prec], std::max<size_t>(n, 1) * 96));
+    if (n) {
+        unsigned blocks = (unsigned)((n + 255) / 256);
+        if (prec == TG_FP64)
+            pack_f64_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
+                                                         raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
+                                                         raw(c, 8), raw(c, 9), raw_charge(c),
+                                                         (RecF64*)c->d_rec[prec]);
+        else
+            pack_mixed_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
+                                                           raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
+                                                           raw(c, 8), raw(c, 9), raw_charge(c),
+                                                           (RecMixed*)c->d_rec[prec]);
+        TG_HIP(c, hipGetLastError());
+    }
+    c->rec_valid[prec] = true;
+    return TG_OK;
+}
+
+bool finite_all(const double* a, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+// Group pixels into tiles of 64 whose arrival-time curves are close, so the
+// wave's lanes walk nearly the same trajectory steps for each window.  The
+// feature of a pixel is the bucket position (t_ret - t0)/dt of the middle
+// particle at 9 points along its trajectory; tiles are the leaves of a k-d
+// split (largest spread first) cut at multiples of 64.
+void kd_split(std::vector<int>& idx, const std::vector<double>& feat, int nf, int begin, int end)
+{
+    int n = end - begin;
+    if (n <= kLanes) return;
+    int tiles = (n + kLanes - 1) / kLanes;
+    int mid = begin + (tiles / 2) * kLanes;
+    int best = 0;
+    double spread = -1.0;
+    for (int d = 0; d < nf; ++d) {
+        double lo = 1e300, hi = -1e300;
+        for (int i = begin; i < end; ++i) {
+            double v = feat[(size_t)idx[i] * nf + d];
+            lo = std::min(lo, v);
+            hi = std::max(hi, v);
+        }
+        if (hi - lo > spread) {
+            spread = hi - lo;
+            best = d;
+        }
+    }
+    std::nth_element(idx.begin() + begin, idx.begin() + mid, idx.begin() + end, [&](int a, int b) {
+        double fa = feat[(size_t)a * nf + best], fb = feat[(size_t)b * nf + best];
+        return fa < fb || (fa == fb && a < b);
+    });
+    kd_split(idx, feat, nf, begin, mid);
+    kd_split(idx, feat, nf, mid, end);
+}
+
+int ensure_tiles(tg_context* c, ScreenHost& s)
+{
+    if (s.perm_valid) return TG_OK;
+    const int nf = 9;
+    std::vector<double> samp(4 * nf, 0.0);
+    if (c->n_particles > 0 && c->n_points > 0) {
+        int p = c->n_particles / 2;
+        for (int j = 0; j < nf; ++j) {
+            size_t k = (size_t)((double)(c->n_points - 1) * j / (nf - 1));
+            size_t g = (size_t)p * c->n_points + k;
+            for (int q = 0; q < 4; ++q)
+                TG_HIP(c, hipMemcpy(&samp[4 * j + q], raw(c, q) + g, sizeof(double), hipMemcpyDeviceToHost));
+        }
+    }
+    std::vector<double> feat((size_t)s.n_pix * nf);
+    for (int i = 0; i < s.n_pix; ++i)
+        for (int j = 0; j < nf; ++j) {
+            double dx = s.obs[3 * i] - samp[4 * j + 1], dy = s.obs[3 * i + 1] - samp[4 * j + 2],
+                   dz = s.obs[3 * i + 2] - samp[4 * j + 3];
+            double ts = samp[4 * j] + std::sqrt(dx * dx + dy * dy + dz * dz) / kSpeedOfLight;
+            feat[(size_t)i * nf + j] = (ts - s.t0[i]) / s.dt;
+        }
+    std::vector<int> idx(s.n_pix);
+    for (int i = 0; i < s.n_pix; ++i) idx[i] = i;
+    kd_split(idx, feat, nf, 0, s.n_pix);
+    s.n_tiles = (s.n_pix + kLanes - 1) / kLanes;
+    std::vector<int> perm((size_t)s.n_tiles * kLanes, -1);
+    for (int i = 0; i < s.n_pix; ++i) perm[i] = idx[i];
+    if (s.d_perm) TG_HIP(c, hipFree(s.d_perm));
+    s.d_perm = nullptr;
+    TG_HIP(c, hipMalloc((void**)&s.d_perm, perm.size() * sizeof(int)));
+    TG_HIP(c, hipMemcpy(s.d_perm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice));
+    s.perm_valid = true;
+    return TG_OK;
+}
+
+int default_window(const tg_context* c, int n_bins)
+{
+    (void)c;
+    return std::max(1, std::min(n_bins, 8));
+}
+
+int max_window_for_lds(const tg_context* c) { return (int)(c->max_lds / (6 * kLanes * sizeof(double))); }
+
+// Build the work list for screens `ids`, particles [p0,p1).
+int make_plan(tg_context* c, const std::vector<int>& ids, int p0, int p1, const tg_tuning* tun,
+              LaunchPlan& plan)
+{
+    plan = LaunchPlan();
+    int np = p1 - p0;
+    long long total_tiles = 0;
+    for (int id : ids) total_tiles += c->screens[id].n_tiles;
+    if (np <= 0 || total_tiles == 0) return TG_OK;
+    int n_chunks;
+    if (tun && tun->particles_per_chunk > 0) {
+        n_chunks = (np + tun->particles_per_chunk - 1) / tun->particles_per_chunk;
+    } else {
+        long long target = 16LL * std::max(c->n_cu, 1);
+        n_chunks = (int)std::max<long long>(1, (target + total_tiles - 1) / total_tiles);
+        n_chunks = std::min(n_chunks, std::max(1, np / 32));
+    }
+    // cap the partial-trace scratch at 48 GB
+    size_t per_chunk = 0;
+    for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
+    const size_t cap = (size_t)48 << 30;
+    while (n_chunks > 1 && (size_t)n_chunks * per_chunk * sizeof(double) > cap) --n_chunks;
+    n_chunks = std::max(1, std::min(n_chunks, np));
+    int chunk = (np + n_chunks - 1) / n_chunks;
+    n_chunks = (np + chunk - 1) / chunk;
+    plan.n_chunks = n_chunks;
+    plan.chunk = chunk;
+    plan.screen_ids = ids;
+    int wmax_lds = max_window_for_lds(c);
+    size_t part_off = 0;
+    for (int id : ids) {
+        ScreenHost& s = c->screens[id];
+        ScreenDev d;
+        d.obs = s.d_obs;
+        d.t0 = s.d_t0;
+        d.perm = s.d_perm;
+        d.partial = nullptr;  // patched once the scratch is allocated
+        d.dt = s.dt;
+        d.inv_dt = 1.0 / s.dt;
+        d.n_pix = s.n_pix;
+        d.n_bins = s.n_bins;
+        d.n_tiles = s.n_tiles;
+        int w = (tun && tun->bins_per_window > 0) ? tun->bins_per_window
+                                                   : (s.window > 0 ? s.window : default_window(c, s.n_bins));
+        w = std::max(1, std::min({w, s.n_bins, wmax_lds}));
+        d.window = w;
+        plan.max_window = std::max(plan.max_window, w);
+        plan.sdev.push_back(d);
+        part_off += (size_t)n_chunks * s.n_pix * s.n_bins * 6;
+    }
+    plan.partial_doubles = part_off;
+    long long cur = 0;
+    for (int ch = 0; ch < n_chunks; ++ch) {
+        int a = p0 + ch * chunk, b = std::min(p1, a + chunk);
+        for (size_t si = 0; si < ids.size(); ++si) {
+            const ScreenHost& s = c->screens[ids[si]];
+            for (int t = 0; t < s.n_tiles; ++t) {
+                WorkItem w;
+                w.screen = (int)si;
+                w.tile = t;
+                w.chunk = ch;
+                w.p0 = a;
+                w.p1 = b;
+                w.cursor_off = cur;
+                cur += b - a;
+                plan.items.push_back(w);
+            }
+        }
+    }
+    plan.cursor_count = (size_t)cur;
+    return TG_OK;
+}
+
+// Allocate scratch and upload the plan; fills plan.sdev[].partial.
+int upload_plan(tg_context* c, LaunchPlan& plan)
+{
+    int rc;
+    if ((rc = ensure_cap(c, c->d_partial, c->cap_partial, plan.partial_doubles))) return rc;
+    if ((rc = ensure_cap(c, c->d_cursors, c->cap_cursors, plan.cursor_count))) return rc;
+    if ((rc = ensure_cap(c, c->d_items, c->cap_items, plan.items.size()))) return rc;
+    if ((rc = ensure_cap(c, c->d_sdev, c->cap_sdev, plan.sdev.size()))) return rc;
+    size_t off = 0;
+    for (size_t i = 0; i < plan.sdev.size(); ++i) {
+        plan.sdev[i].partial = c->d_partial + off;
+        off += (size_t)plan.n_chunks * plan.sdev[i].n_pix * plan.sdev[i].n_bins * 6;
+    }
+    TG_HIP(c, hipMemcpy(c->d_items, plan.items.data(), plan.items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    TG_HIP(c, hipMemcpy(c->d_sdev, plan.sdev.data(), plan.sdev.size() * sizeof(ScreenDev), hipMemcpyHostToDevice));
+    return TG_OK;
+}
+
+// Queue radiation kernel + chunk reductions (capturable: kernels only).
+int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
+{
+    if (plan.items.empty()) return TG_OK;
+    size_t lds = (size_t)plan.max_window * 6 * kLanes * sizeof(double);
+    if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    if (prec == TG_FP64)
+        radiation_kernel<0><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
+            c->d_items, c->d_sdev, (const RecF64*)c->d_rec[0], c->n_points, c->d_cursors);
+    else
+        radiation_kernel<1><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
+            c->d_items, c->d_sdev, (const RecMixed*)c->d_rec[1], c->n_points, c->d_cursors);
+    TG_HIP(c, hipGetLastError());
+    if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    for (size_t i = 0; i < plan.sdev.size(); ++i) {
+        const ScreenHost& s = c->screens[plan.screen_ids[i]];
+        size_t n = (size_t)s.n_pix * s.n_bins * 6;
+        unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+        reduce_partials_kernel<<<blocks, 256, 0, c->stream>>>(plan.sdev[i].partial, s.d_traces, n, plan.n_chunks);
+        TG_HIP(c, hipGetLastError());
+    }
+    if (events) TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    return TG_OK;
+}
+
+int check_ctx(const tg_context* c) { return c ? TG_OK : TG_ERR_ARG; }
+
+int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id, int p0, int p1)
+{
+    if (prec != TG_FP64 && prec != TG_MIXED) return fail(c, TG_ERR_ARG, "precision must be TG_FP64 or TG_MIXED");
+    if (!c->d_raw && c->n_particles == 0 && !c->screens.empty())
+        return fail(c, TG_ERR_STATE, "tg_integrate: no trajectories set");
+    if (screen_id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    if (p0 < 0 || p1 > c->n_particles || p0 > p1) return fail(c, TG_ERR_RANGE, "particle range out of range");
+    c->kernel_ms = c->total_ms = 0.f;
+    // silent no-ops of the reference: nots < 2, NP < 2 (particle.cpp:778,783)
+    if (c->n_points < 2 || p0 == p1) return TG_OK;
+    std::vector<int> ids;
+    for (int i = 0; i < (int)c->screens.size(); ++i)
+        if ((screen_id < 0 || i == screen_id) && c->screens[i].n_bins >= 2 && c->screens[i].n_pix > 0)
+            ids.push_back(i);
+    if (ids.empty()) return TG_OK;
+    int rc;
+    if ((rc = ensure_records(c, prec))) return rc;
+    for (int id : ids)
+        if ((rc = ensure_tiles(c, c->screens[id]))) return rc;
+    tg_tuning eff = {};
+    if (tun)
+        eff = *tun;
+    else if (c->has_tuned[prec])
+        eff = c->tuned[prec];
+    bool use_graph = eff.use_graph != 0;
+    char keybuf[256];
+    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk,
+             screen_id, p0, p1, c->screens.size());
+    std::string key(keybuf);
+    if (use_graph && c->gexec && c->graph_key == key) {
+        TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        TG_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+        TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+        TG_HIP(c, hipEventSynchronize(c->ev[2]));
+        float ms = 0.f;
+        TG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[2]));
+        c->kernel_ms = c->total_ms = ms;
+        return TG_OK;
+    }
+    LaunchPlan plan;
+    if ((rc = make_plan(c, ids, p0, p1, tun ? tun : (c->has_tuned[prec] ? &c->tuned[prec] : nullptr), plan)))
+        return rc;
+    if ((rc = upload_plan(c, plan))) return rc;
+    if (use_graph) {
+        free_graph(c);
+        TG_HIP(c, hipStreamSynchronize(c->stream));
+        TG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_plan(c, prec, plan, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        c->graph = g;
+        TG_HIP(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->graph_key = key;
+        TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        TG_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+        TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+        TG_HIP(c, hipEventSynchronize(c->ev[2]));
+        float ms = 0.f;
+        TG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[2]));
+        c->kernel_ms = c->total_ms = ms;
+        return TG_OK;
+    }
+    if ((rc = enqueue_plan(c, prec, plan, true))) return rc;
+    TG_HIP(c, hipEventSynchronize(c->ev[2]));
+    TG_HIP(c, hipEventElapsedTime(&c->kernel_ms, c->ev[0], c->ev[1]));
+    TG_HIP(c, hipEventElapsedTime(&c->total_ms, c->ev[0], c->ev[2]));
+    return TG_OK;
+}
+
+int set_traj_common(tg_context* c, int n_particles, int n_points, const double* const* src, const double* charge,
+                    hipMemcpyKind kind)
+{
+    if (n_particles < 0 || n_points < 0) return fail(c, TG_ERR_ARG, "negative trajectory size");
+    for (int i = 0; i < kNumRaw; ++i)
+        if (!src[i] && (size_t)n_particles * n_points > 0) return fail(c, TG_ERR_ARG, "null trajectory array");
+    if (!charge && n_particles > 0) return fail(c, TG_ERR_ARG, "null charge array");
+    size_t n = (size_t)n_particles * n_points;
+    if (kind == hipMemcpyHostToDevice) {
+        for (int i = 0; i < kNumRaw; ++i)
+            if (n && !finite_all(src[i], n)) return fail(c, TG_ERR_ARG, "non-finite trajectory value");
+        if (n_particles && !finite_all(charge, n_particles)) return fail(c, TG_ERR_ARG, "non-finite charge");
+    }
+    int rc = alloc_traj(c, n_particles, n_points);
+    if (rc) return rc;
+    for (int i = 0; i < kNumRaw; ++i)
+        if (n) TG_HIP(c, hipMemcpy(raw(c, i), src[i], n * sizeof(double), kind));
+    if (n_particles) TG_HIP(c, hipMemcpy(raw_charge(c), charge, n_particles * sizeof(double), kind));
+    return TG_OK;
+}
+
+}  // namespace
+
+// ============================== C ABI ======================================
+
+extern "C" {
+
+int tg_create(int device, tg_context** out)
+{
+    if (!out) return TG_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return TG_ERR_HIP;
+    if (device < 0 || device >= n) return TG_ERR_ARG;
+    tg_context* c = new (std::nothrow) tg_context;
+    if (!c) return TG_ERR_OOM;
+    c->device = device;
+    auto bad = [&](int rc) {
+        tg_destroy(c);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bad(TG_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bad(TG_ERR_HIP);
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bad(TG_ERR_HIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bad(TG_ERR_HIP);
+    snprintf(c->name, sizeof c->name, "%s %s", prop.name, prop.gcnArchName);
+    c->n_cu = prop.multiProcessorCount;
+    c->max_lds = prop.sharedMemPerBlock;
+    if (hipFuncSetAttribute((const void*)radiation_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)c->max_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void*)radiation_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)c->max_lds) != hipSuccess)
+        (void)hipGetLastError();
+    *out = c;
+    return TG_OK;
+}
+
+int tg_destroy(tg_context* c)
+{
+    if (!c) return TG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_graph(c);
+    (void)free_traj(c);
+    for (auto& s : c->screens) {
+        (void)hipFree(s.d_obs);
+        (void)hipFree(s.d_t0);
+        (void)hipFree(s.d_traces);
+        (void)hipFree(s.d_perm);
+    }
+    (void)hipFree(c->d_items);
+    (void)hipFree(c->d_sdev);
+    (void)hipFree(c->d_cursors);
+    (void)hipFree(c->d_partial);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TG_OK;
+}
+
+const char* tg_last_error(const tg_context* c) { return c ? c->err.c_str() : "null context"; }
+
+int tg_device_info(const tg_context* c, char* name, int name_len, int* n_cu)
+{
+    if (!c) return TG_ERR_ARG;
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
+    if (n_cu) *n_cu = c->n_cu;
+    return TG_OK;
+}
+
+int tg_set_trajectories(tg_context* c, int n_particles, int n_points, const double* t, const double* x,
+                        const double* y, const double* z, const double* ux, const double* uy, const double* uz,
+                        const double* ax, const double* ay, const double* az, const double* charge)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    const double* src[kNumRaw] = {t, x, y, z, ux, uy, uz, ax, ay, az};
+    return set_traj_common(c, n_particles, n_points, src, charge, hipMemcpyHostToDevice);
+}
+
+int tg_set_trajectories_device(tg_context* c, int n_particles, int n_points, const double* t, const double* x,
+                               const double* y, const double* z, const double* ux, const double* uy,
+                               const double* uz, const double* ax, const double* ay, const double* az,
+                               const double* charge)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    const double* src[kNumRaw] = {t, x, y, z, ux, uy, uz, ax, ay, az};
+    return set_traj_common(c, n_particles, n_points, src, charge, hipMemcpyDeviceToDevice);
+}
+
+int tg_trajectory_shape(const tg_context* c, int* n_particles, int* n_points)
+{
+    if (!c) return TG_ERR_ARG;
+    if (n_particles) *n_particles = c->n_particles;
+    if (n_points) *n_points = c->n_points;
+    return TG_OK;
+}
+
+int tg_get_trajectories(tg_context* c, int p_begin, int p_end, double* t, double* x, double* y, double* z,
+                        double* ux, double* uy, double* uz, double* ax, double* ay, double* az, double* charge)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (p_begin < 0 || p_end > c->n_particles || p_begin > p_end) return fail(c, TG_ERR_RANGE, "particle range");
+    TG_HIP(c, hipSetDevice(c->device));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    double* dst[kNumRaw] = {t, x, y, z, ux, uy, uz, ax, ay, az};
+    size_t off = (size_t)p_begin * c->n_points, n = (size_t)(p_end - p_begin) * c->n_points;
+    for (int i = 0; i < kNumRaw; ++i)
+        if (dst[i] && n) TG_HIP(c, hipMemcpy(dst[i], raw(c, i) + off, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (charge && p_end > p_begin)
+        TG_HIP(c, hipMemcpy(charge, raw_charge(c) + p_begin, (p_end - p_begin) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    return TG_OK;
+}
+
+int tg_add_screen(tg_context* c, int n_pix, const double* obs, const double* t0, double dt, int n_bins,
+                  int* screen_id)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (n_pix < 0 || n_bins < 0) return fail(c, TG_ERR_ARG, "negative screen size");
+    if (n_pix > 0 && (!obs || !t0)) return fail(c, TG_ERR_ARG, "null screen array");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, TG_ERR_ARG, "dt must be positive and finite");
+    if (n_pix > 0 && (!finite_all(obs, (size_t)3 * n_pix) || !finite_all(t0, (size_t)n_pix)))
+        return fail(c, TG_ERR_ARG, "non-finite screen geometry");
+    TG_HIP(c, hipSetDevice(c->device));
+    ScreenHost s;
+    s.n_pix = n_pix;
+    s.n_bins = n_bins;
+    s.dt = dt;
+    s.obs.assign(obs, obs + (size_t)3 * n_pix);
+    s.t0.assign(t0, t0 + (size_t)n_pix);
+    size_t nt = (size_t)n_pix * n_bins * 6;
+    TG_HIP(c, hipMalloc((void**)&s.d_obs, std::max<size_t>(1, (size_t)3 * n_pix) * sizeof(double)));
+    TG_HIP(c, hipMalloc((void**)&s.d_t0, std::max<size_t>(1, (size_t)n_pix) * sizeof(double)));
+    TG_HIP(c, hipMalloc((void**)&s.d_traces, std::max<size_t>(1, nt) * sizeof(double)));
+    if (n_pix) {
+        TG_HIP(c, hipMemcpy(s.d_obs, obs, (size_t)3 * n_pix * sizeof(double), hipMemcpyHostToDevice));
+        TG_HIP(c, hipMemcpy(s.d_t0, t0, (size_t)n_pix * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (nt) {
+        fill_kernel<<<(unsigned)std::min<size_t>((nt + 255) / 256, 16384), 256, 0, c->stream>>>(s.d_traces, nt, 0.0);
+        TG_HIP(c, hipGetLastError());
+        TG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->screens.push_back(std::move(s));
+    free_graph(c);
+    if (screen_id) *screen_id = (int)c->screens.size() - 1;
+    return TG_OK;
+}
+
+int tg_zero_screen(tg_context* c, int id)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (id < 0 || id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    TG_HIP(c, hipSetDevice(c->device));
+    ScreenHost& s = c->screens[id];
+    size_t nt = (size_t)s.n_pix * s.n_bins * 6;
+    if (nt) {
+        fill_kernel<<<(unsigned)std::min<size_t>((nt + 255) / 256, 16384), 256, 0, c->stream>>>(s.d_traces, nt, 0.0);
+        TG_HIP(c, hipGetLastError());
+    }
+    return TG_OK;
+}
+
+int tg_remove_screens(tg_context* c)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& s : c->screens) {
+        TG_HIP(c, hipFree(s.d_obs));
+        TG_HIP(c, hipFree(s.d_t0));
+        TG_HIP(c, hipFree(s.d_traces));
+        if (s.d_perm) TG_HIP(c, hipFree(s.d_perm));
+    }
+    c->screens.clear();
+    free_graph(c);
+    return TG_OK;
+}
+
+int tg_screen_count(const tg_context* c, int* n)
+{
+    if (!c || !n) return TG_ERR_ARG;
+    *n = (int)c->screens.size();
+    return TG_OK;
+}
+
+int tg_integrate(tg_context* c, int precision, const tg_tuning* tuning)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    return integrate_impl(c, precision, tuning, -1, 0, c->n_particles);
+}
+
+int tg_integrate_range(tg_context* c, int precision, const tg_tuning* tuning, int screen_id, int p_begin, int p_end)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    return integrate_impl(c, precision, tuning, screen_id, p_begin, p_end);
+}
+
+int tg_synchronize(tg_context* c)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    return TG_OK;
+}
+
+int tg_read_traces(tg_context* c, int id, double* out, int accumulate)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (id < 0 || id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    ScreenHost& s = c->screens[id];
+    size_t nt = (size_t)s.n_pix * s.n_bins * 6;
+    if (nt && !out) return fail(c, TG_ERR_ARG, "null output buffer");
+    TG_HIP(c, hipSetDevice(c->device));
+    TG_HIP(c, hipStreamSynchronize(c->stream));
+    if (!nt) return TG_OK;
+    if (!accumulate) {
+        TG_HIP(c, hipMemcpy(out, s.d_traces, nt * sizeof(double), hipMemcpyDeviceToHost));
+        return TG_OK;
+    }
+    std::vector<double> tmp;
+    try {
+        tmp.resize(nt);
+    } catch (...) {
+        return fail(c, TG_ERR_OOM, "host buffer for accumulate");
+    }
+    TG_HIP(c, hipMemcpy(tmp.data(), s.d_traces, nt * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nt; ++i) out[i] += tmp[i];
+    return TG_OK;
+}
+
+int tg_screen_traces_device(tg_context* c, int id, double** traces)
+{
+    if (check_ctx(c) || !traces) return TG_ERR_ARG;
+    if (id < 0 || id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    *traces = c->screens[id].d_traces;
+    return TG_OK;
+}
+
+int tg_screen_poynting(tg_context* c, int id, double* out)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (id < 0 || id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    ScreenHost& s = c->screens[id];
+    if (s.n_pix == 0) return TG_OK;
+    if (!out) return fail(c, TG_ERR_ARG, "null output buffer");
+    TG_HIP(c, hipSetDevice(c->device));
+    double* d = nullptr;
+    TG_HIP(c, hipMalloc((void**)&d, (size_t)3 * s.n_pix * sizeof(double)));
+    poynting_kernel<<<(s.n_pix + 63) / 64, 64, 0, c->stream>>>(s.d_traces, s.n_pix, s.n_bins, s.dt, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)3 * s.n_pix * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_screen_poynting: ") + hipGetErrorString(e));
+    return TG_OK;
+}
+
+int tg_deposit_only(tg_context* c, int n_pseudo, int n_points, const double* ts, const double* fields, double t0,
+                    double dt, int n_bins, double* trace)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (n_pseudo < 0 || n_points < 0 || n_bins < 0) return fail(c, TG_ERR_ARG, "negative size");
+    if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(t0)) return fail(c, TG_ERR_ARG, "bad t0/dt");
+    if (n_bins < 2 || n_points < 2 || n_pseudo == 0) return TG_OK;  // particle.cpp:778,783
+    if (!ts || !fields || !trace) return fail(c, TG_ERR_ARG, "null pointer");
+    TG_HIP(c, hipSetDevice(c->device));
+    size_t nts = (size_t)n_pseudo * n_points, nf = nts * 6, ntr = (size_t)n_bins * 6;
+    double *dts = nullptr, *df = nullptr, *dtr = nullptr;
+    hipError_t e = hipMalloc((void**)&dts, nts * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&df, nf * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&dtr, ntr * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(dts, ts, nts * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(df, fields, nf * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dtr, trace, ntr * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        deposit_only_kernel<<<1, 64, 0, c->stream>>>(n_pseudo, n_points, dts, df, t0, dt, 1.0 / dt, n_bins, dtr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(trace, dtr, ntr * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dts);
+    (void)hipFree(df);
+    (void)hipFree(dtr);
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? TG_ERR_OOM : TG_ERR_HIP,
+                    std::string("tg_deposit_only: ") + hipGetErrorString(e));
+    return TG_OK;
+}
+
+int tg_last_timing(tg_context* c, float* kernel_ms, float* total_ms)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (kernel_ms) *kernel_ms = c->kernel_ms;
+    if (total_ms) *total_ms = c->total_ms;
+    return TG_OK;
+}
+
+int tg_autotune(tg_context* c, int precision, tg_tuning* out)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (precision != TG_FP64 && precision != TG_MIXED) return fail(c, TG_ERR_ARG, "precision");
+    TG_HIP(c, hipSetDevice(c->device));
+    tg_tuning best = {};
+    if (c->n_particles == 0 || c->n_points < 2 || c->screens.empty()) {
+        c->tuned[precision] = best;
+        c->has_tuned[precision] = true;
+        if (out) *out = best;
+        return TG_OK;
+    }
+    // Time each window size on a particle sample, per screen; keep the
+    // fastest per screen (the launch shape adapts to each screen's bins).
+    const int cands[] = {4, 8, 16, 32};
+    int wmax = max_window_for_lds(c);
+    int sample = std::min(c->n_particles, std::max(64, c->n_particles / 16));
+    std::vector<std::vector<double>> saved(c->screens.size());
+    for (size_t id = 0; id < c->screens.size(); ++id) {
+        ScreenHost& s = c->screens[id];
+        if (s.n_bins < 2 || s.n_pix == 0) continue;
+        size_t nt = (size_t)s.n_pix * s.n_bins * 6;
+        saved[id].resize(nt);
+        TG_HIP(c, hipMemcpy(saved[id].data(), s.d_traces, nt * sizeof(double), hipMemcpyDeviceToHost));
+        float best_ms = 1e30f;
+        int best_w = default_window(c, s.n_bins);
+        for (int w : cands) {
+            if (w > wmax || w > s.n_bins) continue;
+            tg_tuning t = {};
+            t.bins_per_window = w;
+            int rc = integrate_impl(c, precision, &t, (int)id, 0, sample);
+            if (rc) return rc;
+            if (c->kernel_ms < best_ms) {
+                best_ms = c->kernel_ms;
+                best_w = w;
+            }
+        }
+        s.window = best_w;
+        TG_HIP(c, hipMemcpy(s.d_traces, saved[id].data(), nt * sizeof(double), hipMemcpyHostToDevice));
+    }
+    best.bins_per_window = 0;  // per-screen windows stored on the screens
+    c->tuned[precision] = best;
+    c->has_tuned[precision] = true;
+    free_graph(c);
+    if (out) {
+        *out = best;
+        int w = 0;
+        for (auto& s : c->screens) w = std::max(w, s.window);
+        out->bins_per_window = w;
+    }
+    return TG_OK;
+}
+
+int tg_track_undulator(tg_context* c, int n_particles, const double* x0, const double* u0, double t_start,
+                       double charge, double mass, double dt, int n_steps, double bpeak, double lambda_u,
+                       int n_periods, const double* origin)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (n_particles < 0 || n_steps < 0) return fail(c, TG_ERR_ARG, "negative size");
+    if (n_particles > 0 && (!x0 || !u0)) return fail(c, TG_ERR_ARG, "null initial conditions");
+    if (!origin) return fail(c, TG_ERR_ARG, "null origin");
+    if (!(dt > 0.0) || !(lambda_u > 0.0) || !(mass > 0.0)) return fail(c, TG_ERR_ARG, "dt, lambda_u, mass must be > 0");
+    TG_HIP(c, hipSetDevice(c->device));
+    int rc = alloc_traj(c, n_particles, n_steps + 1);
+    if (rc) return rc;
+    if (n_particles == 0) return TG_OK;
+    double* d0 = nullptr;
+    TG_HIP(c, hipMalloc((void**)&d0, (size_t)6 * n_particles * sizeof(double)));
+    hipError_t e = hipMemcpy(d0, x0, (size_t)3 * n_particles * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d0 + 3 * (size_t)n_particles, u0, (size_t)3 * n_particles * sizeof(double),
+                                       hipMemcpyHostToDevice);
+    // charge array
+    std::vector<double> q((size_t)n_particles, charge);
+    if (e == hipSuccess) e = hipMemcpy(raw_charge(c), q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        Undulator u;
+        u.bpeak = bpeak;
+        u.lambda_u = lambda_u;
+        u.kz = 2.0 * kPi / lambda_u;
+        u.z1 = -(n_periods + 1) * lambda_u / 2.0;
+        u.z2 = (n_periods + 1) * lambda_u / 2.0;
+        u.origin = V3{origin[0], origin[1], origin[2]};
+        double qm = charge / mass * kInvRestMass;
+        double qmt2 = qm * 0.5 * dt;
+        vay_undulator_kernel<<<(n_particles + 63) / 64, 64, 0, c->stream>>>(
+            n_particles, d0, d0 + 3 * (size_t)n_particles, t_start, qm, qmt2, dt, n_steps, u, raw(c, 0), raw(c, 1),
+            raw(c, 2), raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7), raw(c, 8), raw(c, 9));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d0);
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_track_undulator: ") + hipGetErrorString(e));
+    return TG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// tg_device.h - device-side data layout and arithmetic of the gfx950
+// radiation kernels.
+//
+// Retarded time and bucket index are computed bit-for-bit as the reference
+// computes them on the host (src/particle.cpp:560-568, 805, 829-830): one ulp
+// of t_ret (~7e-24 s at 33 ns) moves ~1e-10 of a bucket's content to its
+// neighbour, so FP64 parity below 1e-12 needs identical doubles there.  That
+// chain therefore uses only IEEE-rounded operations in the reference's order:
+// FMA contraction off, correctly rounded sqrt, and divisions either as IEEE
+// division or as Markstein's division by a constant (q = a*y, r = fma(-q,b,a),
+// q' = fma(r,y,q) with y = RN(1/b); P. Markstein, IBM J. Res. Dev. 34 (1990)
+// 111-119), which returns RN(a/b).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+// src/global.h:37-44
+constexpr double kElementaryCharge = 1.6021766208e-19;
+constexpr double kSpeedOfLight = 2.99792458e8;
+constexpr double kPi = 3.1415926535897932384626433832795029;
+constexpr double kEpsNull = 8.854187817e-12;
+constexpr double kMuNull = 4 * kPi * 1e-7;
+constexpr double kInvRestMass = 1.758820025e11;
+constexpr double kInvSpeedOfLight = 1.0 / kSpeedOfLight;  // RN(1/c), folded by the compiler
+
+constexpr int kLanes = 64;  // one wavefront = one pixel tile
+
+// Packed trajectory point for the FP64 kernel: 12 doubles = 96 B, read by the
+// whole wave at one uniform address (scalar loads).  beta = u/gamma and
+// beta' = a/gamma are hoisted out of the pixel loop; both are computed exactly
+// as src/particle.cpp:575-582 computes them.
+struct alignas(32) RecF64 {
+    double t, x, y, z;
+    double bx, by, bz;
+    double bpx, bpy, bpz;
+    double g2inv;   // 1/gamma^2
+    double scale;   // q e / (4 pi eps0)
+};
+
+// Packed trajectory point for the mixed kernel (96 B): FP64 where the
+// retarded time and 1-n.beta need it, FP32 amplitude inputs with the charge
+// scale folded in.
+struct alignas(32) RecMixed {
+    double t, x, y, z;
+    double bx, by, bz;
+    float bpx, bpy, bpz;   // beta'
+    float sv;              // scale / gamma^2      (velocity term)
+    float sa;              // scale / c            (acceleration term)
+    float fbx, fby, fbz;   // beta in FP32
+};
+static_assert(sizeof(RecF64) == 96, "RecF64 layout");
+static_assert(sizeof(RecMixed) == 96, "RecMixed layout");
+
+// One screen as the kernel sees it.
+struct ScreenDev {
+    const double* obs;    // [n_pix][3]
+    const double* t0;     // [n_pix]
+    const int* perm;      // [n_tiles*64] pixel index per lane, -1 = padding
+    double* partial;      // [n_chunks][n_pix][n_bins][6]
+    double dt;
+    double inv_dt;        // RN(1/dt)
+    int n_pix;
+    int n_bins;
+    int n_tiles;
+    int window;           // trace bins per LDS window
+};
+
+// One wavefront's work: a pixel tile of one screen and a chunk of particles.
+struct WorkItem {
+    int screen;
+    int tile;
+    int chunk;
+    int p0, p1;           // particle range
+    long long cursor_off; // offset of this item's cursors (p1-p0 int2)
+};
+
+// -------- IEEE-exact helpers (no contraction) ------------------------------
+
+// RN(a/b) for b with y = RN(1/b) precomputed (Markstein).
+__device__ __forceinline__ double div_by_const(double a, double b, double y)
+{
+    double q = a * y;
+    double r = __builtin_fma(-q, b, a);
+    return __builtin_fma(r, y, q);
+}
+
+// round() of the reference (half away from zero), clamped into int range;
+// the clamp only touches values far outside every trace, where the reference
+// never evaluates the index.
+__device__ __forceinline__ int round_index(double q)
+{
+    double r = __builtin_round(q);
+    r = __builtin_fmin(__builtin_fmax(r, -1.0e9), 1.0e9);
+    return (int)r;
+}
+
+}  // namespace tg
