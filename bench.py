@@ -59,14 +59,16 @@ def build_workload(ctx, name, n_particles=None):
 
 
 def time_integrate(ctx, prec, steps, warmup, tuning=None):
-    for _ in range(warmup):
+    for i in range(warmup):
         ctx.integrate(prec, tuning)
+        log("  warmup %d/%d: %.3f s" % (i + 1, warmup, ctx.last_timing()[1] / 1e3))
     ctx.synchronize()
     kms = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         ctx.integrate(prec, tuning)
         kms.append(ctx.last_timing())
+        log("  step %d/%d: %.3f s" % (i + 1, steps, kms[-1][1] / 1e3))
     ctx.synchronize()
     wall = (time.perf_counter() - t0) / steps
     return wall, float(np.mean([k for k, _ in kms])) / 1e3, float(np.mean([t for _, t in kms])) / 1e3
@@ -153,6 +155,11 @@ def main():
     value = n_pairs / wall
     log("%s: %.3f s/step (kernel %.3f s, kernel+reduce %.3f s) -> %.4g pairs/s"
         % (args.precision, wall, kern, total, value))
+    st = ctx.last_stats()
+    if st["work_items"]:
+        ideal = n_pairs / 64.0
+        log("walk stats: %s; wave-steps / ideal = %.3f, steps per walk = %.1f"
+            % (st, st["wave_steps"] / ideal, st["wave_steps"] / max(1, st["walks"])))
     peak = mixed_peak() if prec == TG_MIXED else PEAK_FP64
     achieved = n_pairs * FLOP_PER_PAIR / kern
     extra = {}

@@ -141,6 +141,13 @@ int tg_autotune(tg_context* ctx, int precision, tg_tuning* out);
  * the whole launch sequence, measured with HIP events on the context's stream. */
 int tg_last_timing(tg_context* ctx, float* kernel_ms, float* total_ms);
 
+/* Diagnostic with no counterpart in the reference: when the environment has
+ * TG_STATS=1 at tg_create, the last tg_integrate's radiation kernel counts the
+ * trajectory steps each wavefront evaluated (wave_steps, 64 pixels each), the
+ * particle walks (one per work item, window and contributing particle) and
+ * the work items; otherwise all three read 0. */
+int tg_last_stats(tg_context* ctx, long long* wave_steps, long long* walks, long long* work_items);
+
 /* Trajectory generation on the device (the tracking step that precedes the
  * radiation step): Vay pusher (J.-L. Vay 2008) through a planar undulator,
  * ChargedParticle::InitVay/StepVay (src/particle.cpp:407-512) with

@@ -155,6 +155,11 @@ class RadiationContext:
         self._chk(self.lib.tg_last_timing(self.h, ctypes.byref(k), ctypes.byref(t)))
         return k.value, t.value
 
+    def last_stats(self):
+        v = [ctypes.c_longlong() for _ in range(3)]
+        self._chk(self.lib.tg_last_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return {"wave_steps": v[0].value, "walks": v[1].value, "work_items": v[2].value}
+
     def autotune(self, precision=TG_MIXED):
         t = TgTuning()
         self._chk(self.lib.tg_autotune(self.h, precision, ctypes.byref(t)))

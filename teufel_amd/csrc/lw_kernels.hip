@@ -15,9 +15,10 @@
 // particle order and step order, so the deposit needs no atomics and is
 // bitwise reproducible.  Per (work item, particle) a cursor remembers the
 // first step that reaches the next window, so every step is walked about
-// once per work item.  Trajectory points are uniform across the wave and are
-// read with scalar loads.  Chunks write partial traces that a second kernel
-// sums in fixed chunk order into the screen's trace store.
+// once per work item.  Trajectory points are uniform across the wave; they are
+// staged through LDS in 16-record chunks loaded one chunk ahead.  Particle
+// chunks write partial traces that a second kernel sums in fixed chunk order
+// into the screen's trace store.
 #include "tg_device.h"
 
 #include <climits>
@@ -112,7 +113,7 @@ __device__ __forceinline__ void timing_exact(double t, double dx, double dy, dou
 {
 #pragma clang fp contract(off)
     double r2 = dx * dx + dy * dy + dz * dz;
-    R = __builtin_sqrt(r2);
+    R = sqrt_rn(r2);
     ts = t + div_by_const(R, kSpeedOfLight, kInvSpeedOfLight);
     idx = round_index(div_by_const(ts - t0, dt, inv_dt));
 }
@@ -131,7 +132,7 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
         dy = Py - r.y;
         dz = Pz - r.z;
         timing_exact(r.t, dx, dy, dz, t0, dt, inv_dt, R, o.ts, o.idx);
-        invR = 1.0 / R;
+        invR = rcp_rn(R);
         nx = div_by_const(dx, R, invR);
         ny = div_by_const(dy, R, invR);
         nz = div_by_const(dz, R, invR);
@@ -140,7 +141,7 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
     }
     double mx = nx - r.bx, my = ny - r.by, mz = nz - r.bz;   // n - beta
     double k3 = kappa * kappa * kappa;
-    double irk3 = 1.0 / (R * k3);                              // 1/(R kappa^3)
+    double irk3 = rcp_rn(R * k3);                             // 1/(R kappa^3)
     double cv = r.g2inv * invR * irk3;                        // 1/(R^2 kappa^3 gamma^2)
     double ca = irk3 * kInvSpeedOfLight;                      // 1/(R kappa^3 c)
     double nbp = nx * r.bpx + ny * r.bpy + nz * r.bpz;        // n . beta'
@@ -239,10 +240,10 @@ __device__ __forceinline__ void deposit_step(const Acc& acc, bool active, double
     if (!caseA && idx1 == i2) {
         // step within one bucket: (f1+f2)*0.5*(dts/dt)
         double w = 0.5 * (dts * inv_dt);
-        acc(idx1, w, w);
+        acc.same(idx1, w);
         return;
     }
-    const double idts = 1.0 / dts;
+    const double idts = rcp_rn(dts);
     for (int j = jlo; j <= jhi; ++j) {
         double a, b;
         if (!caseA && j == idx1) {
@@ -272,6 +273,13 @@ __device__ __forceinline__ void deposit_step(const Acc& acc, bool active, double
 }
 
 // LDS window accumulator: cell (bin j, component c, lane) at ((j-b0)*6+c)*64+lane.
+// Each cell has exactly one writer (its lane), which adds in program order, so
+// the non-returning LDS add (ds_add_f64) is deterministic; it needs no wait.
+__device__ __forceinline__ void lds_add(double* p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 struct LdsAcc {
     double* base;  // lds + lane
     int b0;
@@ -281,7 +289,13 @@ struct LdsAcc {
     {
         double* cell = base + (j - b0) * (6 * kLanes);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) cell[c * kLanes] += a * f1[c] + b * f2[c];
+        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, a * f1[c] + b * f2[c]);
+    }
+    __device__ __forceinline__ void same(int j, double w) const
+    {
+        double* cell = base + (j - b0) * (6 * kLanes);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, w * (f1[c] + f2[c]));
     }
 };
 
@@ -295,30 +309,94 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
+// Trajectory records of the particle being walked are staged through LDS in
+// chunks of 16 (1536 B): each lane loads 24 B of a chunk with ordinary
+// coalesced loads one chunk ahead of use, and the wave reads the record of the
+// current step from LDS (same address for all lanes).  The wait for a chunk
+// therefore falls once per 16 steps instead of once per step.
+constexpr int kChunkRecs = 16;
+constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
+
+template <class Rec>
+struct Stager {
+    const double* __restrict__ base;  // the particle's records, as doubles
+    double* slots;                    // LDS: 2 chunks
+    int lane;
+    int lim;                          // last legal 3-double group start (12*NP - 3)
+    int next_fill;                    // chunk held in (q0,q1,q2)
+    double q0, q1, q2;
+
+    __device__ __forceinline__ void load(int chunk, double& a, double& b, double& c) const
+    {
+        int i = kChunkDoubles * chunk + 3 * lane;
+        i = i < lim ? i : lim;
+        a = base[i];
+        b = base[i + 1];
+        c = base[i + 2];
+    }
+    __device__ __forceinline__ void store(int chunk, double a, double b, double c) const
+    {
+        double* d = slots + (chunk & 1) * kChunkDoubles + 3 * lane;
+        d[0] = a;
+        d[1] = b;
+        d[2] = c;
+    }
+    // Stage the chunks holding records k and k+1 and request the next one.
+    __device__ __forceinline__ void start(int k)
+    {
+        const int c0 = k / kChunkRecs;
+        double a, b, c;
+        load(c0, a, b, c);
+        double d, e, f;
+        load(c0 + 1, d, e, f);
+        load(c0 + 2, q0, q1, q2);
+        store(c0, a, b, c);
+        store(c0 + 1, d, e, f);
+        next_fill = c0 + 2;
+    }
+    // Record r (called with r increasing by one per step).
+    __device__ __forceinline__ const Rec& rec(int r)
+    {
+        const int cr = r / kChunkRecs;
+        if (cr == next_fill) {
+            store(next_fill, q0, q1, q2);
+            load(next_fill + 1, q0, q1, q2);
+            ++next_fill;
+        }
+        return *reinterpret_cast<const Rec*>(slots + (cr & 1) * kChunkDoubles + (r % kChunkRecs) * 12);
+    }
+};
+
 // Walk the steps of one particle starting at step k (uniform) for the window
 // [b0, b1).  Returns (through nk/nmin) the first step that reaches bins >= b1
 // for some lane and the smallest start bin of that step over the lanes.
 template <int PREC>
-__device__ __forceinline__ void walk_particle(const typename RecOf<PREC>::type* __restrict__ rp, int k,
-                                              int NP, bool active, double Px, double Py, double Pz,
-                                              double t0, double dt, double inv_dt, double tmin,
-                                              double tmax, int n_bins, int b0, int b1,
-                                              double* lds_lane, int& nk, int& nmin)
+__device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* __restrict__ rp, int k,
+                                             int NP, bool active, double Px, double Py, double Pz,
+                                             double t0, double dt, double inv_dt, double tmin,
+                                             double tmax, int n_bins, int b0, int b1,
+                                             double* lds_lane, double* stage, int lane, int& nk, int& nmin)
 {
+    using Rec = typename RecOf<PREC>::type;
+    Stager<Rec> sg{reinterpret_cast<const double*>(rp), stage, lane, 12 * NP - 3, 0, 0.0, 0.0, 0.0};
+    sg.start(k);
     Point p1, p2;
-    eval_point<PREC>(rp[k], Px, Py, Pz, t0, dt, inv_dt, p1);
+    eval_point<PREC>(sg.rec(k), Px, Py, Pz, t0, dt, inv_dt, p1);
     int next_k = -1, next_min = INT_MAX;
+    int steps = 0;
     while (true) {
         if (!__ballot(active && p1.idx < b1)) {
+            // every lane is past the window: this and later steps belong to later windows
             if (next_k < 0) {
                 next_k = k;
                 next_min = wave_min(active ? p1.idx : INT_MAX);
             }
             break;
         }
-        eval_point<PREC>(rp[k + 1], Px, Py, Pz, t0, dt, inv_dt, p2);
+        eval_point<PREC>(sg.rec(k + 1), Px, Py, Pz, t0, dt, inv_dt, p2);
         LdsAcc acc{lds_lane, b0, p1.f, p2.f};
         deposit_step(acc, active, p1.ts, p1.idx, p2.ts, p2.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        ++steps;
         if (next_k < 0 && __ballot(active && p2.idx >= b1)) {
             next_k = k;
             next_min = wave_min(active ? p1.idx : INT_MAX);
@@ -335,13 +413,16 @@ __device__ __forceinline__ void walk_particle(const typename RecOf<PREC>::type* 
     }
     nk = next_k;
     nmin = next_min;
+    return steps;
 }
 
 template <int PREC>
 __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items,
                                                         const ScreenDev* __restrict__ screens,
                                                         const typename RecOf<PREC>::type* __restrict__ recs,
-                                                        int n_points, int2* __restrict__ cursors)
+                                                        int n_points, int2* __restrict__ cursors,
+                                                        unsigned long long* __restrict__ stats,
+                                                        int lds_window)
 {
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
@@ -362,6 +443,8 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
     }
     int2* __restrict__ cur_base = cursors + it.cursor_off;
     double* lds_lane = lds + lane;
+    double* stage = lds + (size_t)lds_window * 6 * kLanes;   // 2 record chunks after the window
+    unsigned long long n_steps = 0, n_walks = 0;
     for (int b0 = 0; b0 < nb; b0 += W) {
         const int b1 = b0 + W < nb ? b0 + W : nb;
         const int ncell = (b1 - b0) * 6;
@@ -375,11 +458,13 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
             while (todo) {
                 const int j = __ffsll((long long)todo) - 1;
                 todo &= todo - 1;
-                const int k = __builtin_amdgcn_readfirstlane(__shfl(cur.x, j));
+                const int k = __builtin_amdgcn_readlane(cur.x, j);
                 const int p = pb + j;
                 int nk, nmin;
-                walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
-                                    tmin, tmax, nb, b0, b1, lds_lane, nk, nmin);
+                n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
+                                               inv_dt, tmin, tmax, nb, b0, b1, lds_lane, stage, lane,
+                                               nk, nmin);
+                ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
             if (b1 < nb && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
@@ -389,10 +474,16 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
             for (int j = 0; j < ncell; ++j) out[j] = lds_lane[j * kLanes];
         }
     }
+    if (stats && lane == 0) {
+        stats[2 * (size_t)blockIdx.x] = n_steps;
+        stats[2 * (size_t)blockIdx.x + 1] = n_walks;
+    }
 }
 
-template __global__ void radiation_kernel<0>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*);
-template __global__ void radiation_kernel<1>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*);
+template __global__ void radiation_kernel<0>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
+                                             unsigned long long*, int);
+template __global__ void radiation_kernel<1>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
+                                             unsigned long long*, int);
 
 // ---------------------------------------------------------------------------
 // Sum the chunk partials into the trace store, chunks in order.
@@ -426,6 +517,11 @@ struct GlobalAcc {
     {
         double* cell = trace + 6 * j;
         for (int c = 0; c < 6; ++c) cell[c] += a * f1[c] + b * f2[c];
+    }
+    __device__ __forceinline__ void same(int j, double w) const
+    {
+        double* cell = trace + 6 * j;
+        for (int c = 0; c < 6; ++c) cell[c] += w * (f1[c] + f2[c]);
     }
 };
 

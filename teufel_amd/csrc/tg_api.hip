@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -72,6 +73,11 @@ struct tg_context {
     size_t cap_cursors = 0;
     double* d_partial = nullptr;
     size_t cap_partial = 0;
+    // walk statistics (diagnostic; on when TG_STATS=1 in the environment)
+    bool stats_on = false;
+    unsigned long long* d_stats = nullptr;
+    size_t cap_stats = 0;
+    long long last_steps = 0, last_walks = 0, last_items = 0;
     // timing
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float kernel_ms = 0.f, total_ms = 0.f;
@@ -263,7 +269,10 @@ int default_window(const tg_context* c, int n_bins)
     return std::max(1, std::min(n_bins, 8));
 }
 
-int max_window_for_lds(const tg_context* c) { return (int)(c->max_lds / (6 * kLanes * sizeof(double))); }
+int max_window_for_lds(const tg_context* c)
+{
+    return (int)((c->max_lds - 2 * kChunkDoubles * sizeof(double)) / (6 * kLanes * sizeof(double)));
+}
 
 // Build the work list for screens `ids`, particles [p0,p1).
 int make_plan(tg_context* c, const std::vector<int>& ids, int p0, int p1, const tg_tuning* tun,
@@ -346,6 +355,7 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
     if ((rc = ensure_cap(c, c->d_cursors, c->cap_cursors, plan.cursor_count))) return rc;
     if ((rc = ensure_cap(c, c->d_items, c->cap_items, plan.items.size()))) return rc;
     if ((rc = ensure_cap(c, c->d_sdev, c->cap_sdev, plan.sdev.size()))) return rc;
+    if (c->stats_on && (rc = ensure_cap(c, c->d_stats, c->cap_stats, 2 * plan.items.size()))) return rc;
     size_t off = 0;
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
         plan.sdev[i].partial = c->d_partial + off;
@@ -360,14 +370,16 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = (size_t)plan.max_window * 6 * kLanes * sizeof(double);
+    size_t lds = ((size_t)plan.max_window * 6 * kLanes + 2 * kChunkDoubles) * sizeof(double);
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (prec == TG_FP64)
         radiation_kernel<0><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
-            c->d_items, c->d_sdev, (const RecF64*)c->d_rec[0], c->n_points, c->d_cursors);
+            c->d_items, c->d_sdev, (const RecF64*)c->d_rec[0], c->n_points, c->d_cursors,
+            c->stats_on ? c->d_stats : nullptr, plan.max_window);
     else
         radiation_kernel<1><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
-            c->d_items, c->d_sdev, (const RecMixed*)c->d_rec[1], c->n_points, c->d_cursors);
+            c->d_items, c->d_sdev, (const RecMixed*)c->d_rec[1], c->n_points, c->d_cursors,
+            c->stats_on ? c->d_stats : nullptr, plan.max_window);
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
@@ -454,6 +466,16 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
     TG_HIP(c, hipEventSynchronize(c->ev[2]));
     TG_HIP(c, hipEventElapsedTime(&c->kernel_ms, c->ev[0], c->ev[1]));
     TG_HIP(c, hipEventElapsedTime(&c->total_ms, c->ev[0], c->ev[2]));
+    if (c->stats_on) {
+        std::vector<unsigned long long> st(2 * plan.items.size());
+        TG_HIP(c, hipMemcpy(st.data(), c->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        c->last_steps = c->last_walks = 0;
+        for (size_t i = 0; i < plan.items.size(); ++i) {
+            c->last_steps += (long long)st[2 * i];
+            c->last_walks += (long long)st[2 * i + 1];
+        }
+        c->last_items = (long long)plan.items.size();
+    }
     return TG_OK;
 }
 
@@ -507,6 +529,8 @@ int tg_create(int device, tg_context** out)
     snprintf(c->name, sizeof c->name, "%s %s", prop.name, prop.gcnArchName);
     c->n_cu = prop.multiProcessorCount;
     c->max_lds = prop.sharedMemPerBlock;
+    const char* st = getenv("TG_STATS");
+    c->stats_on = st && st[0] == '1';
     if (hipFuncSetAttribute((const void*)radiation_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)c->max_lds) != hipSuccess ||
         hipFuncSetAttribute((const void*)radiation_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -533,6 +557,7 @@ int tg_destroy(tg_context* c)
     (void)hipFree(c->d_sdev);
     (void)hipFree(c->d_cursors);
     (void)hipFree(c->d_partial);
+    (void)hipFree(c->d_stats);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -779,6 +804,15 @@ int tg_last_timing(tg_context* c, float* kernel_ms, float* total_ms)
     if (check_ctx(c)) return TG_ERR_ARG;
     if (kernel_ms) *kernel_ms = c->kernel_ms;
     if (total_ms) *total_ms = c->total_ms;
+    return TG_OK;
+}
+
+int tg_last_stats(tg_context* c, long long* wave_steps, long long* walks, long long* work_items)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (wave_steps) *wave_steps = c->last_steps;
+    if (walks) *walks = c->last_walks;
+    if (work_items) *work_items = c->last_items;
     return TG_OK;
 }
 

@@ -87,6 +87,34 @@ __device__ __forceinline__ double div_by_const(double a, double b, double y)
     return __builtin_fma(r, y, q);
 }
 
+// RN(1/b) from the hardware estimate and two Newton steps with FMA: the last
+// step rounds correctly (Markstein) for the normal, finite b used here.
+__device__ __forceinline__ double rcp_rn(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// RN(sqrt(x)) for normal x > 0 far from the denormal range: Goldschmidt
+// iteration from the hardware 1/sqrt estimate, then two FMA residual
+// corrections; the last one rounds correctly (the sequence compilers use
+// for IEEE sqrt, without their range scaling).
+__device__ __forceinline__ double sqrt_rn(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = __builtin_fma(-h, g, 0.5);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
 // round() of the reference (half away from zero), clamped into int range;
 // the clamp only touches values far outside every trace, where the reference
 // never evaluates the index.
