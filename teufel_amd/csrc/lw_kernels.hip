@@ -100,11 +100,14 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
 // One trajectory point seen from one pixel.
 // ---------------------------------------------------------------------------
 
-struct Point {
-    double ts;     // retarded (arrival) time, bit-identical to the reference
-    int idx;       // round((ts - t0)/dt), bit-identical to the reference
-    double f[6];   // Ex, Ey, Ez, Bx, By, Bz
+template <class F>
+struct PointT {
+    double ts;     // retarded (arrival) time
+    int idx;       // round((ts - t0)/dt)
+    F f[6];        // Ex, Ey, Ez, Bx, By, Bz
 };
+using Point = PointT<double>;
+using PointF = PointT<float>;
 
 // Retarded time t + |P - X|/c and bucket index: src/particle.cpp:560-568,
 // 805, 829-830, in the reference's operation order.
@@ -162,7 +165,7 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
 // Mixed precision: FP64 distance, retarded time and 1 - n.beta (the two
 // quantities whose cancellation FP32 cannot carry), FP32 amplitudes.
 __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double Py, double Pz,
-                                           double t0, double dt, double inv_dt, Point& o)
+                                           double t0, double dt, double inv_dt, PointF& o)
 {
     double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
     double r2 = dx * dx + dy * dy + dz * dz;
@@ -198,16 +201,21 @@ __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double 
     o.f[5] = (fnx * ey - fny * ex) * ic;
 }
 
+// Per precision: trajectory record, amplitude type (also the LDS window type).
+// FP64 accumulates the window in FP64 LDS; the mixed kernel accumulates an
+// FP32 LDS window that is flushed into the FP64 partial trace after every
+// batch of 64 particles, which bounds each FP32 sum to a few hundred terms.
 template <int PREC>
 struct RecOf;
 template <>
-struct RecOf<0> { using type = RecF64; };
+struct RecOf<0> { using type = RecF64; using amp = double; };
 template <>
-struct RecOf<1> { using type = RecMixed; };
+struct RecOf<1> { using type = RecMixed; using amp = float; };
 
 template <int PREC>
 __device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, double Px, double Py,
-                                           double Pz, double t0, double dt, double inv_dt, Point& o)
+                                           double Pz, double t0, double dt, double inv_dt,
+                                           PointT<typename RecOf<PREC>::amp>& o)
 {
     if constexpr (PREC == 0)
         eval_f64(r, Px, Py, Pz, t0, dt, inv_dt, o);
@@ -280,22 +288,30 @@ __device__ __forceinline__ void lds_add(double* p, double v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+__device__ __forceinline__ void lds_add(float* p, float v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <class F>
 struct LdsAcc {
-    double* base;  // lds + lane
+    F* base;  // lds window + lane
     int b0;
-    const double* f1;
-    const double* f2;
+    const F* f1;
+    const F* f2;
     __device__ __forceinline__ void operator()(int j, double a, double b) const
     {
-        double* cell = base + (j - b0) * (6 * kLanes);
+        F* cell = base + (j - b0) * (6 * kLanes);
+        const F fa = (F)a, fb = (F)b;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, a * f1[c] + b * f2[c]);
+        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, fa * f1[c] + fb * f2[c]);
     }
     __device__ __forceinline__ void same(int j, double w) const
     {
-        double* cell = base + (j - b0) * (6 * kLanes);
+        F* cell = base + (j - b0) * (6 * kLanes);
+        const F fw = (F)w;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, w * (f1[c] + f2[c]));
+        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, fw * (f1[c] + f2[c]));
     }
 };
 
@@ -375,12 +391,14 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
                                              int NP, bool active, double Px, double Py, double Pz,
                                              double t0, double dt, double inv_dt, double tmin,
                                              double tmax, int n_bins, int b0, int b1,
-                                             double* lds_lane, double* stage, int lane, int& nk, int& nmin)
+                                             typename RecOf<PREC>::amp* win_lane, double* stage, int lane,
+                                             int& nk, int& nmin)
 {
     using Rec = typename RecOf<PREC>::type;
+    using F = typename RecOf<PREC>::amp;
     Stager<Rec> sg{reinterpret_cast<const double*>(rp), stage, lane, 12 * NP - 3, 0, 0.0, 0.0, 0.0};
     sg.start(k);
-    Point p1, p2;
+    PointT<F> p1, p2;
     eval_point<PREC>(sg.rec(k), Px, Py, Pz, t0, dt, inv_dt, p1);
     int next_k = -1, next_min = INT_MAX;
     int steps = 0;
@@ -394,7 +412,7 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
             break;
         }
         eval_point<PREC>(sg.rec(k + 1), Px, Py, Pz, t0, dt, inv_dt, p2);
-        LdsAcc acc{lds_lane, b0, p1.f, p2.f};
+        LdsAcc<F> acc{win_lane, b0, p1.f, p2.f};
         deposit_step(acc, active, p1.ts, p1.idx, p2.ts, p2.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
         if (next_k < 0 && __ballot(active && p2.idx >= b1)) {
@@ -424,6 +442,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                                                         unsigned long long* __restrict__ stats,
                                                         int lds_window)
 {
+    using F = typename RecOf<PREC>::amp;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const WorkItem it = items[blockIdx.x];
@@ -442,19 +461,23 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         tmax = t0 + dt * ((double)nb - 0.5);
     }
     int2* __restrict__ cur_base = cursors + it.cursor_off;
-    double* lds_lane = lds + lane;
-    double* stage = lds + (size_t)lds_window * 6 * kLanes;   // 2 record chunks after the window
+    F* win_lane = reinterpret_cast<F*>(lds) + lane;
+    // 2 record chunks after the window (window bytes are a multiple of 1536)
+    double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)lds_window * 6 * kLanes);
     unsigned long long n_steps = 0, n_walks = 0;
     for (int b0 = 0; b0 < nb; b0 += W) {
         const int b1 = b0 + W < nb ? b0 + W : nb;
         const int ncell = (b1 - b0) * 6;
-        for (int j = 0; j < ncell; ++j) lds_lane[j * kLanes] = 0.0;
+        double* out = S.partial + (((size_t)it.chunk * S.n_pix + (active ? pix : 0)) * nb + b0) * 6;
+        for (int j = 0; j < ncell; ++j) win_lane[j * kLanes] = (F)0;
+        bool first_flush = true;
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
             const int npb = it.p1 - pb < kLanes ? it.p1 - pb : kLanes;
             int2 cur = make_int2(0, INT_MIN);
             if (b0 > 0 && lane < npb) cur = cur_base[pb - it.p0 + lane];
             unsigned long long todo = __ballot(lane < npb && cur.x < NP - 1 && cur.y < b1);
             int2 ncur = cur;
+            const bool any = todo != 0;
             while (todo) {
                 const int j = __ffsll((long long)todo) - 1;
                 todo &= todo - 1;
@@ -462,16 +485,29 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                 const int p = pb + j;
                 int nk, nmin;
                 n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
-                                               inv_dt, tmin, tmax, nb, b0, b1, lds_lane, stage, lane,
+                                               inv_dt, tmin, tmax, nb, b0, b1, win_lane, stage, lane,
                                                nk, nmin);
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
             if (b1 < nb && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
+            if constexpr (PREC == 1) {
+                // FP32 window -> FP64 partial trace after each batch of particles
+                if (any) {
+                    if (active) {
+                        for (int j = 0; j < ncell; ++j) {
+                            double v = (double)win_lane[j * kLanes];
+                            out[j] = first_flush ? v : out[j] + v;
+                        }
+                    }
+                    for (int j = 0; j < ncell; ++j) win_lane[j * kLanes] = (F)0;
+                    first_flush = false;
+                }
+            }
         }
-        if (active) {
-            double* out = S.partial + (((size_t)it.chunk * S.n_pix + pix) * nb + b0) * 6;
-            for (int j = 0; j < ncell; ++j) out[j] = lds_lane[j * kLanes];
+        if (PREC == 0 || first_flush) {
+            if (active)
+                for (int j = 0; j < ncell; ++j) out[j] = (double)win_lane[j * kLanes];
         }
     }
     if (stats && lane == 0) {

@@ -34,7 +34,7 @@ struct ScreenHost {
     int* d_perm = nullptr;
     int n_tiles = 0;
     bool perm_valid = false;
-    int window = 0;  // autotuned bins per window (0 = default)
+    int window[2] = {0, 0};  // autotuned bins per window per precision (0 = default)
 };
 
 struct LaunchPlan {
@@ -269,13 +269,15 @@ int default_window(const tg_context* c, int n_bins)
     return std::max(1, std::min(n_bins, 8));
 }
 
-int max_window_for_lds(const tg_context* c)
+size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
+
+int max_window_for_lds(const tg_context* c, int prec)
 {
-    return (int)((c->max_lds - 2 * kChunkDoubles * sizeof(double)) / (6 * kLanes * sizeof(double)));
+    return (int)((c->max_lds - 2 * kChunkDoubles * sizeof(double)) / (6 * kLanes * cell_bytes(prec)));
 }
 
 // Build the work list for screens `ids`, particles [p0,p1).
-int make_plan(tg_context* c, const std::vector<int>& ids, int p0, int p1, const tg_tuning* tun,
+int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int p1, const tg_tuning* tun,
               LaunchPlan& plan)
 {
     plan = LaunchPlan();
@@ -302,7 +304,7 @@ int make_plan(tg_context* c, const std::vector<int>& ids, int p0, int p1, const 
     plan.n_chunks = n_chunks;
     plan.chunk = chunk;
     plan.screen_ids = ids;
-    int wmax_lds = max_window_for_lds(c);
+    int wmax_lds = max_window_for_lds(c, prec);
     size_t part_off = 0;
     for (int id : ids) {
         ScreenHost& s = c->screens[id];
@@ -317,7 +319,7 @@ int make_plan(tg_context* c, const std::vector<int>& ids, int p0, int p1, const 
         d.n_bins = s.n_bins;
         d.n_tiles = s.n_tiles;
         int w = (tun && tun->bins_per_window > 0) ? tun->bins_per_window
-                                                   : (s.window > 0 ? s.window : default_window(c, s.n_bins));
+                                                   : (s.window[prec] > 0 ? s.window[prec] : default_window(c, s.n_bins));
         w = std::max(1, std::min({w, s.n_bins, wmax_lds}));
         d.window = w;
         plan.max_window = std::max(plan.max_window, w);
@@ -370,7 +372,7 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = ((size_t)plan.max_window * 6 * kLanes + 2 * kChunkDoubles) * sizeof(double);
+    size_t lds = (size_t)plan.max_window * 6 * kLanes * cell_bytes(prec) + 2 * kChunkDoubles * sizeof(double);
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (prec == TG_FP64)
         radiation_kernel<0><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
@@ -435,7 +437,7 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         return TG_OK;
     }
     LaunchPlan plan;
-    if ((rc = make_plan(c, ids, p0, p1, tun ? tun : (c->has_tuned[prec] ? &c->tuned[prec] : nullptr), plan)))
+    if ((rc = make_plan(c, prec, ids, p0, p1, tun ? tun : (c->has_tuned[prec] ? &c->tuned[prec] : nullptr), plan)))
         return rc;
     if ((rc = upload_plan(c, plan))) return rc;
     if (use_graph) {
@@ -830,8 +832,8 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     }
     // Time each window size on a particle sample, per screen; keep the
     // fastest per screen (the launch shape adapts to each screen's bins).
-    const int cands[] = {4, 8, 16, 32};
-    int wmax = max_window_for_lds(c);
+    const int cands[] = {4, 6, 8, 12, 16, 24, 32};
+    int wmax = max_window_for_lds(c, precision);
     int sample = std::min(c->n_particles, std::max(64, c->n_particles / 16));
     std::vector<std::vector<double>> saved(c->screens.size());
     for (size_t id = 0; id < c->screens.size(); ++id) {
@@ -853,7 +855,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
                 best_w = w;
             }
         }
-        s.window = best_w;
+        s.window[precision] = best_w;
         TG_HIP(c, hipMemcpy(s.d_traces, saved[id].data(), nt * sizeof(double), hipMemcpyHostToDevice));
     }
     best.bins_per_window = 0;  // per-screen windows stored on the screens
@@ -863,7 +865,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     if (out) {
         *out = best;
         int w = 0;
-        for (auto& s : c->screens) w = std::max(w, s.window);
+        for (auto& s : c->screens) w = std::max(w, s.window[precision]);
         out->bins_per_window = w;
     }
     return TG_OK;
