@@ -225,9 +225,14 @@ __device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, 
 
 // ---------------------------------------------------------------------------
 // The bucket deposit of one trajectory step, src/particle.cpp:788-865,
-// restricted to bins [b0, b1).  Every contribution is alpha*f1 + beta*f2;
-// time arithmetic (bucket edges/centres) is done as the reference does it.
-// acc(j, alpha, beta) adds the contribution into bin j.
+// restricted to bins [b0, b1).  The reference's three cases are one rule per
+// bin j of the step's bin range: the covered interval [left, right] is the
+// step clipped to the bucket edges t0 + (j -/+ 1/2) dt; a fully covered bucket
+// takes the field at its centre t0 + j dt, a partially covered one the mean of
+// the interpolated end values times (right - left)/dt.  Every time value is
+// formed exactly as the reference forms it (contraction off), so only the
+// amplitude weights differ from the reference, by rounding.  acc(j, a, b)
+// adds a*f1 + b*f2 into bin j.
 // ---------------------------------------------------------------------------
 template <class Acc>
 __device__ __forceinline__ void deposit_step(const Acc& acc, bool active, double ts1, int idx1,
@@ -236,47 +241,31 @@ __device__ __forceinline__ void deposit_step(const Acc& acc, bool active, double
                                              int b0, int b1)
 {
 #pragma clang fp contract(off)
-    if (!(active && ts2 > tmin && ts1 < tmax)) return;
     const bool caseA = ts1 <= tmin;
     const int i2 = idx2 > n_bins ? n_bins : idx2;
     const int lo = caseA ? 0 : idx1;
     const int hi = i2 < n_bins - 1 ? i2 : n_bins - 1;
-    const int jlo = lo > b0 ? lo : b0;
-    const int jhi = hi < b1 - 1 ? hi : b1 - 1;
+    int jlo = lo > b0 ? lo : b0;
+    int jhi = hi < b1 - 1 ? hi : b1 - 1;
+    if (!(active && ts2 > tmin && ts1 < tmax)) jhi = jlo - 1;
     if (jlo > jhi) return;
     const double dts = ts2 - ts1;
-    if (!caseA && idx1 == i2) {
-        // step within one bucket: (f1+f2)*0.5*(dts/dt)
-        double w = 0.5 * (dts * inv_dt);
-        acc.same(idx1, w);
-        return;
-    }
     const double idts = rcp_rn(dts);
     for (int j = jlo; j <= jhi; ++j) {
-        double a, b;
-        if (!caseA && j == idx1) {
-            // first, partially covered bucket
-            double t_end = t0 + (idx1 + 0.5) * dt;
-            double ae = (ts2 - t_end) * idts;
-            double be = (t_end - ts1) * idts;
-            double h = 0.5 * ((t_end - ts1) * inv_dt);
-            a = h * (1.0 + ae);
-            b = h * be;
-        } else if (j == i2) {
-            // last, partially covered bucket (i2 < n_bins here)
-            double t_start = t0 + (i2 - 0.5) * dt;
-            double as = (ts2 - t_start) * idts;
-            double bs = (t_start - ts1) * idts;
-            double h = 0.5 * ((ts2 - t_start) * inv_dt);
-            a = h * as;
-            b = h * (bs + 1.0);
-        } else {
-            // fully covered bucket: value at the bucket centre
-            double tc = t0 + j * dt;
-            a = (ts2 - tc) * idts;
-            b = (tc - ts1) * idts;
-        }
-        acc(j, a, b);
+        const bool first = !caseA && j == idx1;       // step starts inside bucket j
+        const bool last = j == i2;                    // step ends inside bucket j (i2 < n_bins)
+        const double left = first ? ts1 : t0 + (j - 0.5) * dt;
+        const double right = last ? ts2 : t0 + (j + 0.5) * dt;
+        const double tc = t0 + j * dt;
+        // partially covered: 0.5*(right-left)/dt * (f(left) + f(right))
+        const double h = 0.5 * ((right - left) * inv_dt);
+        const double ap = h * ((ts2 - left) * idts + (ts2 - right) * idts);
+        const double bp = h * ((left - ts1) * idts + (right - ts1) * idts);
+        // fully covered: f(centre)
+        const double af = (ts2 - tc) * idts;
+        const double bf = (tc - ts1) * idts;
+        const bool full = !(first || last);
+        acc(j, full ? af : ap, full ? bf : bp);
     }
 }
 
@@ -370,22 +359,27 @@ struct Stager {
         store(c0 + 1, d, e, f);
         next_fill = c0 + 2;
     }
-    // Record r (called with r increasing by one per step).
-    __device__ __forceinline__ const Rec& rec(int r)
+    // Make records up to r available (r advances by at most 2 per call, so
+    // at most one chunk is crossed).
+    __device__ __forceinline__ void prepare(int r)
     {
-        const int cr = r / kChunkRecs;
-        if (cr == next_fill) {
+        if (r / kChunkRecs == next_fill) {
             store(next_fill, q0, q1, q2);
             load(next_fill + 1, q0, q1, q2);
             ++next_fill;
         }
-        return *reinterpret_cast<const Rec*>(slots + (cr & 1) * kChunkDoubles + (r % kChunkRecs) * 12);
+    }
+    __device__ __forceinline__ const Rec& at(int r) const
+    {
+        return *reinterpret_cast<const Rec*>(slots + ((r / kChunkRecs) & 1) * kChunkDoubles + (r % kChunkRecs) * 12);
     }
 };
 
 // Walk the steps of one particle starting at step k (uniform) for the window
 // [b0, b1).  Returns (through nk/nmin) the first step that reaches bins >= b1
 // for some lane and the smallest start bin of that step over the lanes.
+// Two trajectory points are evaluated per iteration as independent chains, so
+// a single wave keeps the SIMD busy while each chain waits on its latencies.
 template <int PREC>
 __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* __restrict__ rp, int k,
                                              int NP, bool active, double Px, double Py, double Pz,
@@ -398,28 +392,57 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
     using F = typename RecOf<PREC>::amp;
     Stager<Rec> sg{reinterpret_cast<const double*>(rp), stage, lane, 12 * NP - 3, 0, 0.0, 0.0, 0.0};
     sg.start(k);
-    PointT<F> p1, p2;
-    eval_point<PREC>(sg.rec(k), Px, Py, Pz, t0, dt, inv_dt, p1);
+    PointT<F> p0, q1, q2;
+    eval_point<PREC>(sg.at(k), Px, Py, Pz, t0, dt, inv_dt, p0);
     int next_k = -1, next_min = INT_MAX;
     int steps = 0;
     while (true) {
-        if (!__ballot(active && p1.idx < b1)) {
+        if (!__ballot(active && p0.idx < b1)) {
             // every lane is past the window: this and later steps belong to later windows
             if (next_k < 0) {
                 next_k = k;
-                next_min = wave_min(active ? p1.idx : INT_MAX);
+                next_min = wave_min(active ? p0.idx : INT_MAX);
             }
             break;
         }
-        eval_point<PREC>(sg.rec(k + 1), Px, Py, Pz, t0, dt, inv_dt, p2);
-        LdsAcc<F> acc{win_lane, b0, p1.f, p2.f};
-        deposit_step(acc, active, p1.ts, p1.idx, p2.ts, p2.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
-        ++steps;
-        if (next_k < 0 && __ballot(active && p2.idx >= b1)) {
-            next_k = k;
-            next_min = wave_min(active ? p1.idx : INT_MAX);
+        const int r2 = k + 2 < NP ? k + 2 : NP - 1;
+        sg.prepare(r2);
+        eval_point<PREC>(sg.at(k + 1), Px, Py, Pz, t0, dt, inv_dt, q1);
+        eval_point<PREC>(sg.at(r2), Px, Py, Pz, t0, dt, inv_dt, q2);
+        {
+            LdsAcc<F> acc{win_lane, b0, p0.f, q1.f};
+            deposit_step(acc, active, p0.ts, p0.idx, q1.ts, q1.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         }
-        ++k;
+        ++steps;
+        if (next_k < 0 && __ballot(active && q1.idx >= b1)) {
+            next_k = k;
+            next_min = wave_min(active ? p0.idx : INT_MAX);
+        }
+        // second step (k+1, k+2) if it exists and some lane is still in the window
+        const bool two = (k + 1 < NP - 1) && __ballot(active && q1.idx < b1);
+        if (!two) {
+            ++k;
+            if (k >= NP - 1) {
+                if (next_k < 0) {
+                    next_k = NP - 1;
+                    next_min = INT_MAX;
+                }
+            } else if (next_k < 0) {
+                next_k = k;
+                next_min = wave_min(active ? q1.idx : INT_MAX);
+            }
+            break;
+        }
+        {
+            LdsAcc<F> acc{win_lane, b0, q1.f, q2.f};
+            deposit_step(acc, active, q1.ts, q1.idx, q2.ts, q2.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        }
+        ++steps;
+        if (next_k < 0 && __ballot(active && q2.idx >= b1)) {
+            next_k = k + 1;
+            next_min = wave_min(active ? q1.idx : INT_MAX);
+        }
+        k += 2;
         if (k >= NP - 1) {
             if (next_k < 0) {
                 next_k = NP - 1;
@@ -427,7 +450,7 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
             }
             break;
         }
-        p1 = p2;
+        p0 = q2;
     }
     nk = next_k;
     nmin = next_min;

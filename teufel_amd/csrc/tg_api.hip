@@ -832,9 +832,9 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     }
     // Time each window size on a particle sample, per screen; keep the
     // fastest per screen (the launch shape adapts to each screen's bins).
-    const int cands[] = {4, 6, 8, 12, 16, 24, 32};
+    const int cands[] = {6, 8, 12, 16, 24};
     int wmax = max_window_for_lds(c, precision);
-    int sample = std::min(c->n_particles, std::max(64, c->n_particles / 16));
+    int sample = std::min(c->n_particles, std::max(64, c->n_particles / 32));
     std::vector<std::vector<double>> saved(c->screens.size());
     for (size_t id = 0; id < c->screens.size(); ++id) {
         ScreenHost& s = c->screens[id];
