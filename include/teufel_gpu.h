@@ -56,7 +56,7 @@ typedef struct {
     int bins_per_window;     /* trace bins held in LDS per work item: 4, 8, 16 or 32 */
     int particles_per_chunk; /* particles per work item (partial traces summed in order) */
     int use_graph;           /* 1: replay the launch sequence from a captured hipGraph */
-    int reserved;
+    int window_groups;       /* windows of a (tile, chunk) split into this many work items */
 } tg_tuning;
 
 /* Context on one device; one trace store and one trajectory set per context.

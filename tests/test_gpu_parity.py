@@ -97,15 +97,16 @@ def _small_bunch(n_particles=48, n_steps=4000, npx=16, n_bins=256, seed=77):
     return tr, obs, t0, dt, nrm
 
 
-@pytest.mark.parametrize("window,chunk", [(8, 0), (4, 7), (16, 1000), (32, 16)])
-def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk):
-    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d" % (window, chunk))
+@pytest.mark.parametrize("window,chunk,groups", [(8, 0, 1), (4, 7, 3), (16, 1000, 2), (32, 16, 5), (6, 0, 0)])
+def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk, groups):
+    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d groups=%d" % (window, chunk, groups))
     tr, obs, t0, dt, _ = _small_bunch()
     ref = oracle.integrate_screen(tr, obs, t0, dt, 256)
     ctx.remove_screens()
     ctx.set_trajectories(tr)
     sid = ctx.add_screen(obs, t0, dt, 256)
-    ctx.integrate(TG_FP64, tuning={"bins_per_window": window, "particles_per_chunk": chunk})
+    ctx.integrate(TG_FP64, tuning={"bins_per_window": window, "particles_per_chunk": chunk,
+                                   "window_groups": groups})
     err = trace_rel_error(ctx.read_traces(sid), ref)
     print("eps_E = %.3e" % err)
     assert err <= FP64_TOL

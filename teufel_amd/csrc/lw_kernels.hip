@@ -20,6 +20,7 @@
 // chunks write partial traces that a second kernel sums in fixed chunk order
 // into the screen's trace store.
 #include "tg_device.h"
+#include "tg_deposit.h"
 
 #include <climits>
 
@@ -223,51 +224,10 @@ __device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, 
         eval_mixed(r, Px, Py, Pz, t0, dt, inv_dt, o);
 }
 
-// ---------------------------------------------------------------------------
-// The bucket deposit of one trajectory step, src/particle.cpp:788-865,
-// restricted to bins [b0, b1).  The reference's three cases are one rule per
-// bin j of the step's bin range: the covered interval [left, right] is the
-// step clipped to the bucket edges t0 + (j -/+ 1/2) dt; a fully covered bucket
-// takes the field at its centre t0 + j dt, a partially covered one the mean of
-// the interpolated end values times (right - left)/dt.  Every time value is
-// formed exactly as the reference forms it (contraction off), so only the
-// amplitude weights differ from the reference, by rounding.  acc(j, a, b)
-// adds a*f1 + b*f2 into bin j.
-// ---------------------------------------------------------------------------
-template <class Acc>
-__device__ __forceinline__ void deposit_step(const Acc& acc, bool active, double ts1, int idx1,
-                                             double ts2, int idx2, double t0, double dt,
-                                             double inv_dt, double tmin, double tmax, int n_bins,
-                                             int b0, int b1)
-{
-#pragma clang fp contract(off)
-    const bool caseA = ts1 <= tmin;
-    const int i2 = idx2 > n_bins ? n_bins : idx2;
-    const int lo = caseA ? 0 : idx1;
-    const int hi = i2 < n_bins - 1 ? i2 : n_bins - 1;
-    int jlo = lo > b0 ? lo : b0;
-    int jhi = hi < b1 - 1 ? hi : b1 - 1;
-    if (!(active && ts2 > tmin && ts1 < tmax)) jhi = jlo - 1;
-    if (jlo > jhi) return;
-    const double dts = ts2 - ts1;
-    const double idts = rcp_rn(dts);
-    for (int j = jlo; j <= jhi; ++j) {
-        const bool first = !caseA && j == idx1;       // step starts inside bucket j
-        const bool last = j == i2;                    // step ends inside bucket j (i2 < n_bins)
-        const double left = first ? ts1 : t0 + (j - 0.5) * dt;
-        const double right = last ? ts2 : t0 + (j + 0.5) * dt;
-        const double tc = t0 + j * dt;
-        // partially covered: 0.5*(right-left)/dt * (f(left) + f(right))
-        const double h = 0.5 * ((right - left) * inv_dt);
-        const double ap = h * ((ts2 - left) * idts + (ts2 - right) * idts);
-        const double bp = h * ((left - ts1) * idts + (right - ts1) * idts);
-        // fully covered: f(centre)
-        const double af = (ts2 - tc) * idts;
-        const double bf = (tc - ts1) * idts;
-        const bool full = !(first || last);
-        acc(j, full ? af : ap, full ? bf : bp);
-    }
-}
+// Reciprocal used by the deposit weights (amplitude-level, not timing).
+struct RcpRn {
+    __device__ __forceinline__ double operator()(double x) const { return rcp_rn(x); }
+};
 
 // LDS window accumulator: cell (bin j, component c, lane) at ((j-b0)*6+c)*64+lane.
 // Each cell has exactly one writer (its lane), which adds in program order, so
@@ -294,13 +254,6 @@ struct LdsAcc {
         const F fa = (F)a, fb = (F)b;
 #pragma unroll
         for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, fa * f1[c] + fb * f2[c]);
-    }
-    __device__ __forceinline__ void same(int j, double w) const
-    {
-        F* cell = base + (j - b0) * (6 * kLanes);
-        const F fw = (F)w;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, fw * (f1[c] + f2[c]));
     }
 };
 
@@ -386,7 +339,7 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
                                              double t0, double dt, double inv_dt, double tmin,
                                              double tmax, int n_bins, int b0, int b1,
                                              typename RecOf<PREC>::amp* win_lane, double* stage, int lane,
-                                             int& nk, int& nmin)
+                                             bool verify, int& nk, int& nmin)
 {
     using Rec = typename RecOf<PREC>::type;
     using F = typename RecOf<PREC>::amp;
@@ -394,6 +347,17 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
     sg.start(k);
     PointT<F> p0, q1, q2;
     eval_point<PREC>(sg.at(k), Px, Py, Pz, t0, dt, inv_dt, p0);
+    if (verify) {
+        // a searched cursor is a guess: step back until no lane starts inside
+        // or beyond the window at point k, so no earlier step reaches it
+        int back = 32;
+        while (k > 0 && __ballot(active && p0.idx >= b0)) {
+            k = k > back ? k - back : 0;
+            back *= 2;
+            sg.start(k);
+            eval_point<PREC>(sg.at(k), Px, Py, Pz, t0, dt, inv_dt, p0);
+        }
+    }
     int next_k = -1, next_min = INT_MAX;
     int steps = 0;
     while (true) {
@@ -411,7 +375,8 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
         eval_point<PREC>(sg.at(r2), Px, Py, Pz, t0, dt, inv_dt, q2);
         {
             LdsAcc<F> acc{win_lane, b0, p0.f, q1.f};
-            deposit_step(acc, active, p0.ts, p0.idx, q1.ts, q1.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+            deposit_step(acc, RcpRn{}, active, p0.ts, p0.idx, q1.ts, q1.idx, t0, dt, inv_dt, tmin, tmax, n_bins,
+                         b0, b1);
         }
         ++steps;
         if (next_k < 0 && __ballot(active && q1.idx >= b1)) {
@@ -435,7 +400,8 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
         }
         {
             LdsAcc<F> acc{win_lane, b0, q1.f, q2.f};
-            deposit_step(acc, active, q1.ts, q1.idx, q2.ts, q2.idx, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+            deposit_step(acc, RcpRn{}, active, q1.ts, q1.idx, q2.ts, q2.idx, t0, dt, inv_dt, tmin, tmax, n_bins,
+                         b0, b1);
         }
         ++steps;
         if (next_k < 0 && __ballot(active && q2.idx >= b1)) {
@@ -457,6 +423,29 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
     return steps;
 }
 
+// First trajectory point of each particle of a batch whose bucket for one
+// representative pixel is >= b: lanes = particles, binary search over the
+// monotone retarded time.  The result seeds a cursor that the walk verifies.
+template <int PREC>
+__device__ __forceinline__ int search_first_point(const typename RecOf<PREC>::type* __restrict__ rp_lane, int NP,
+                                                  double Px, double Py, double Pz, double t0, double dt,
+                                                  double inv_dt, int b)
+{
+    int lo = 0, hi = NP - 1;   // invariant: answer in [lo, hi]; hi may not satisfy
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const auto& r = rp_lane[mid];
+        double R, ts;
+        int idx;
+        timing_exact(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, dt, inv_dt, R, ts, idx);
+        if (idx >= b)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
 template <int PREC>
 __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items,
                                                         const ScreenDev* __restrict__ screens,
@@ -466,6 +455,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                                                         int lds_window)
 {
     using F = typename RecOf<PREC>::amp;
+    constexpr int kFlushBatches = 16;   // mixed: FP32 window flushed every 16 x 64 particles
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const WorkItem it = items[blockIdx.x];
@@ -488,35 +478,50 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
     // 2 record chunks after the window (window bytes are a multiple of 1536)
     double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)lds_window * 6 * kLanes);
     unsigned long long n_steps = 0, n_walks = 0;
-    for (int b0 = 0; b0 < nb; b0 += W) {
-        const int b1 = b0 + W < nb ? b0 + W : nb;
+    // seed the cursors of a group that does not start at bucket 0
+    if (it.bin_lo > 0) {
+        const int l0 = S.perm[it.tile * kLanes];
+        const double Rx = S.obs[3 * l0], Ry = S.obs[3 * l0 + 1], Rz = S.obs[3 * l0 + 2], Rt0 = S.t0[l0];
+        for (int pb = it.p0; pb < it.p1; pb += kLanes) {
+            const int p = pb + lane;
+            if (p < it.p1) {
+                int j = search_first_point<PREC>(recs + (size_t)p * NP, NP, Rx, Ry, Rz, Rt0, dt, inv_dt, it.bin_lo);
+                int k = j > 0 ? j - 1 : 0;
+                cur_base[p - it.p0] = make_int2(k < NP - 1 ? k : NP - 1, INT_MIN);
+            }
+        }
+    }
+    for (int b0 = it.bin_lo; b0 < it.bin_hi; b0 += W) {
+        const int b1 = b0 + W < it.bin_hi ? b0 + W : it.bin_hi;
         const int ncell = (b1 - b0) * 6;
         double* out = S.partial + (((size_t)it.chunk * S.n_pix + (active ? pix : 0)) * nb + b0) * 6;
         for (int j = 0; j < ncell; ++j) win_lane[j * kLanes] = (F)0;
         bool first_flush = true;
+        int pending = 0;
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
             const int npb = it.p1 - pb < kLanes ? it.p1 - pb : kLanes;
             int2 cur = make_int2(0, INT_MIN);
             if (b0 > 0 && lane < npb) cur = cur_base[pb - it.p0 + lane];
             unsigned long long todo = __ballot(lane < npb && cur.x < NP - 1 && cur.y < b1);
             int2 ncur = cur;
-            const bool any = todo != 0;
+            if (todo) ++pending;
             while (todo) {
                 const int j = __ffsll((long long)todo) - 1;
                 todo &= todo - 1;
                 const int k = __builtin_amdgcn_readlane(cur.x, j);
+                const bool verify = __builtin_amdgcn_readlane(cur.y, j) == INT_MIN && k > 0;
                 const int p = pb + j;
                 int nk, nmin;
                 n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
                                                inv_dt, tmin, tmax, nb, b0, b1, win_lane, stage, lane,
-                                               nk, nmin);
+                                               verify, nk, nmin);
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
-            if (b1 < nb && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
+            if (b1 < it.bin_hi && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
             if constexpr (PREC == 1) {
-                // FP32 window -> FP64 partial trace after each batch of particles
-                if (any) {
+                // FP32 window -> FP64 partial trace every kFlushBatches batches with work
+                if (pending >= kFlushBatches) {
                     if (active) {
                         for (int j = 0; j < ncell; ++j) {
                             double v = (double)win_lane[j * kLanes];
@@ -525,12 +530,16 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                     }
                     for (int j = 0; j < ncell; ++j) win_lane[j * kLanes] = (F)0;
                     first_flush = false;
+                    pending = 0;
                 }
             }
         }
-        if (PREC == 0 || first_flush) {
-            if (active)
+        if (active) {
+            if (PREC == 0 || first_flush) {
                 for (int j = 0; j < ncell; ++j) out[j] = (double)win_lane[j * kLanes];
+            } else if (pending > 0) {
+                for (int j = 0; j < ncell; ++j) out[j] += (double)win_lane[j * kLanes];
+            }
         }
     }
     if (stats && lane == 0) {
@@ -577,11 +586,6 @@ struct GlobalAcc {
         double* cell = trace + 6 * j;
         for (int c = 0; c < 6; ++c) cell[c] += a * f1[c] + b * f2[c];
     }
-    __device__ __forceinline__ void same(int j, double w) const
-    {
-        double* cell = trace + 6 * j;
-        for (int c = 0; c < 6; ++c) cell[c] += w * (f1[c] + f2[c]);
-    }
 };
 
 __global__ void deposit_only_kernel(int n_pseudo, int n_points, const double* __restrict__ ts,
@@ -603,7 +607,7 @@ __global__ void deposit_only_kernel(int n_pseudo, int n_points, const double* __
             int i1 = round_index(div_by_const(ts1 - t0, dt, inv_dt));
             int i2 = round_index(div_by_const(ts2 - t0, dt, inv_dt));
             GlobalAcc acc{trace, fp + 6 * k, fp + 6 * (k + 1)};
-            deposit_step(acc, true, ts1, i1, ts2, i2, t0, dt, inv_dt, tmin, tmax, n_bins, 0, n_bins);
+            deposit_step(acc, RcpRn{}, true, ts1, i1, ts2, i2, t0, dt, inv_dt, tmin, tmax, n_bins, 0, n_bins);
         }
     }
 }

@@ -46,6 +46,7 @@ struct LaunchPlan {
     size_t partial_doubles = 0;
     size_t cursor_count = 0;
     int max_window = 0;
+    int groups = 1;
 };
 
 }  // namespace
@@ -327,21 +328,37 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
         part_off += (size_t)n_chunks * s.n_pix * s.n_bins * 6;
     }
     plan.partial_doubles = part_off;
+    // Window groups: each (tile, chunk) is split into groups of consecutive
+    // windows so that there are several times more work items than wave
+    // slots; a group seeds its cursors by search instead of walking from 0.
+    long long base_items = total_tiles * n_chunks;
+    long long slots = 8LL * std::max(c->n_cu, 1);
+    int groups = (int)std::max<long long>(1, std::min<long long>(16, (4 * slots + base_items - 1) / base_items));
+    if (tun && tun->window_groups > 0) groups = tun->window_groups;
+    plan.groups = groups;
     long long cur = 0;
     for (int ch = 0; ch < n_chunks; ++ch) {
         int a = p0 + ch * chunk, b = std::min(p1, a + chunk);
         for (size_t si = 0; si < ids.size(); ++si) {
             const ScreenHost& s = c->screens[ids[si]];
-            for (int t = 0; t < s.n_tiles; ++t) {
-                WorkItem w;
-                w.screen = (int)si;
-                w.tile = t;
-                w.chunk = ch;
-                w.p0 = a;
-                w.p1 = b;
-                w.cursor_off = cur;
-                cur += b - a;
-                plan.items.push_back(w);
+            const int W = plan.sdev[si].window;
+            int nw = (s.n_bins + W - 1) / W;
+            int g_n = std::min(groups, nw);
+            for (int g = 0; g < g_n; ++g) {
+                int w_lo = (int)((long long)nw * g / g_n), w_hi = (int)((long long)nw * (g + 1) / g_n);
+                for (int t = 0; t < s.n_tiles; ++t) {
+                    WorkItem w;
+                    w.screen = (int)si;
+                    w.tile = t;
+                    w.chunk = ch;
+                    w.p0 = a;
+                    w.p1 = b;
+                    w.bin_lo = w_lo * W;
+                    w.bin_hi = std::min(s.n_bins, w_hi * W);
+                    w.cursor_off = cur;
+                    cur += b - a;
+                    plan.items.push_back(w);
+                }
             }
         }
     }
@@ -423,7 +440,7 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         eff = c->tuned[prec];
     bool use_graph = eff.use_graph != 0;
     char keybuf[256];
-    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk,
+    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk, eff.window_groups,
              screen_id, p0, p1, c->screens.size());
     std::string key(keybuf);
     if (use_graph && c->gexec && c->graph_key == key) {

@@ -74,6 +74,7 @@ struct WorkItem {
     int tile;
     int chunk;
     int p0, p1;           // particle range
+    int bin_lo, bin_hi;   // bucket range [bin_lo, bin_hi) of this item (a group of windows)
     long long cursor_off; // offset of this item's cursors (p1-p0 int2)
 };
 
