@@ -52,11 +52,13 @@ SIGNATURES = [
 _lib = None
 
 
-def load(path=LIB_PATH):
-    """Load the HIP library; raises if it has not been built."""
+def load(path=None):
+    """Load the HIP library; raises if it has not been built.  TEUFEL_AMD_LIB
+    in the environment names another build of the same library."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("TEUFEL_AMD_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise RuntimeError(
             "teufel_amd: HIP library %s is not built (run __graft_entry__.build() "
