@@ -230,18 +230,9 @@ struct RcpRn {
 };
 
 // LDS window accumulator: cell (bin j, component c, lane) at ((j-b0)*6+c)*64+lane.
-// Each cell has exactly one writer (its lane), which adds in program order, so
-// the non-returning LDS add (ds_add_f64) is deterministic; it needs no wait.
-__device__ __forceinline__ void lds_add(double* p, double v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-__device__ __forceinline__ void lds_add(float* p, float v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
+// Each cell has exactly one writer (its own lane), and a wave's LDS accesses
+// execute in order, so a plain read-add-write is race-free and deterministic
+// (LDS float atomics serialise per lane and are much slower here).
 template <class F>
 struct LdsAcc {
     F* base;  // lds window + lane
@@ -252,8 +243,11 @@ struct LdsAcc {
     {
         F* cell = base + (j - b0) * (6 * kLanes);
         const F fa = (F)a, fb = (F)b;
+        F v[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) lds_add(cell + c * kLanes, fa * f1[c] + fb * f2[c]);
+        for (int c = 0; c < 6; ++c) v[c] = cell[c * kLanes];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cell[c * kLanes] = v[c] + (fa * f1[c] + fb * f2[c]);
     }
 };
 
