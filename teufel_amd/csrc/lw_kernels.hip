@@ -15,10 +15,10 @@
 // particle order and step order, so the deposit needs no atomics and is
 // bitwise reproducible.  Per (work item, particle) a cursor remembers the
 // first step that reaches the next window, so every step is walked about
-// once per work item.  Trajectory points are uniform across the wave; they are
-// staged through LDS in 16-record chunks loaded one chunk ahead.  Particle
-// chunks write partial traces that a second kernel sums in fixed chunk order
-// into the screen's trace store.
+// once per work item.  Trajectory points are uniform across the wave and are
+// read with scalar loads one step ahead of use.  Particle chunks write partial
+// traces that a second kernel sums in fixed chunk order into the screen's
+// trace store.
 #include "tg_device.h"
 #include "tg_deposit.h"
 
@@ -68,7 +68,7 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
                                   const double* __restrict__ uy, const double* __restrict__ uz,
                                   const double* __restrict__ ax, const double* __restrict__ ay,
                                   const double* __restrict__ az, const double* __restrict__ charge,
-                                  RecMixed* __restrict__ out)
+                                  RecMixed* __restrict__ out, int* __restrict__ off_axis)
 {
 #pragma clang fp contract(off)
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,18 +83,18 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
     r.x = x[g];
     r.y = y[g];
     r.z = z[g];
-    r.bx = ux[g] / gamma;
-    r.by = uy[g] / gamma;
     r.bz = uz[g] / gamma;
-    r.bpx = (float)(ax[g] / gamma);
-    r.bpy = (float)(ay[g] / gamma);
-    r.bpz = (float)(az[g] / gamma);
+    double bx = ux[g] / gamma, by = uy[g] / gamma;
+    r.bx = (float)bx;
+    r.by = (float)by;
+    double sa = scale / kSpeedOfLight;
+    r.apx = (float)(ax[g] / gamma * sa);
+    r.apy = (float)(ay[g] / gamma * sa);
+    r.apz = (float)(az[g] / gamma * sa);
     r.sv = (float)(scale / gamma2);
-    r.sa = (float)(scale / kSpeedOfLight);
-    r.fbx = (float)r.bx;
-    r.fby = (float)r.by;
-    r.fbz = (float)r.bz;
     out[g] = r;
+    // the FP32 transverse beta needs |beta_x| + |beta_y| << 1 (see RecMixed)
+    if (__builtin_fabs(bx) + __builtin_fabs(by) > 0.25) *off_axis = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -163,43 +163,40 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
     o.f[5] = (nx * ey - ny * ex) * kInvSpeedOfLight;
 }
 
-// Mixed precision: FP64 distance, retarded time and 1 - n.beta (the two
-// quantities whose cancellation FP32 cannot carry), FP32 amplitudes.
+// Mixed precision: FP64 distance, retarded time and the n_z beta_z part of
+// 1 - n.beta (the cancelling term), FP32 for everything else.
 __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double Py, double Pz,
-                                           double t0, double dt, double inv_dt, PointF& o)
+                                           double t0, double dt, double inv_dt, PointT<float>& o)
 {
-    double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
-    double r2 = dx * dx + dy * dy + dz * dz;
+    const double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
     // 1/sqrt: FP32 hardware estimate + one FP64 Newton step (~2^-46 relative)
     double y = (double)__builtin_amdgcn_rsqf((float)r2);
-    double h = r2 * y;
-    double e = __builtin_fma(-h, y, 1.0);
+    const double h = r2 * y;
+    const double e = __builtin_fma(-h, y, 1.0);
     y = __builtin_fma(0.5 * y, e, y);
-    double R = r2 * y;
+    const double R = r2 * y;
     o.ts = __builtin_fma(R, kInvSpeedOfLight, r.t);
     o.idx = round_index((o.ts - t0) * inv_dt);
-    double nx = dx * y, ny = dy * y, nz = dz * y;
-    double kappa = 1.0 - (r.bx * nx + r.by * ny + r.bz * nz);
-    float fnx = (float)nx, fny = (float)ny, fnz = (float)nz;
-    float mx = (float)(nx - r.bx), my = (float)(ny - r.by), mz = (float)(nz - r.bz);
-    float fk = (float)kappa;
-    float fy = (float)y;
-    float irk3 = fy * __builtin_amdgcn_rcpf(fk * fk * fk);
-    float cv = r.sv * fy * irk3;
-    float ca = r.sa * irk3;
-    float nbp = fnx * r.bpx + fny * r.bpy + fnz * r.bpz;
-    float c1 = cv + ca * nbp;
-    float c2 = ca * fk;                                       // n.(n-beta) = kappa for unit n
-    float ex = mx * c1 - r.bpx * c2;
-    float ey = my * c1 - r.bpy * c2;
-    float ez = mz * c1 - r.bpz * c2;
+    const double nz = dz * y;
+    const float fy = (float)y;
+    const float nx = (float)dx * fy, ny = (float)dy * fy, fnz = (float)nz;
+    const float kappa = (float)__builtin_fma(-nz, r.bz, 1.0) - (nx * r.bx + ny * r.by);
+    const float mx = nx - r.bx, my = ny - r.by, mz = (float)(nz - r.bz);
+    const float irk3 = fy * __builtin_amdgcn_rcpf(kappa * kappa * kappa);   // 1/(R kappa^3)
+    const float cv = r.sv * fy * irk3;                                       // scale/(R^2 kappa^3 gamma^2)
+    const float c1 = cv + irk3 * (nx * r.apx + ny * r.apy + fnz * r.apz);
+    const float c2 = irk3 * kappa;
+    const float ex = mx * c1 - r.apx * c2;
+    const float ey = my * c1 - r.apy * c2;
+    const float ez = mz * c1 - r.apz * c2;
     const float ic = (float)kInvSpeedOfLight;
     o.f[0] = ex;
     o.f[1] = ey;
     o.f[2] = ez;
-    o.f[3] = (fny * ez - fnz * ey) * ic;
-    o.f[4] = (fnz * ex - fnx * ez) * ic;
-    o.f[5] = (fnx * ey - fny * ex) * ic;
+    o.f[3] = (ny * ez - fnz * ey) * ic;
+    o.f[4] = (fnz * ex - nx * ez) * ic;
+    o.f[5] = (nx * ey - ny * ex) * ic;
 }
 
 // Per precision: trajectory record, amplitude type (also the LDS window type).
@@ -224,30 +221,59 @@ __device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, 
         eval_mixed(r, Px, Py, Pz, t0, dt, inv_dt, o);
 }
 
-// Reciprocal used by the deposit weights (amplitude-level, not timing).
+// Reciprocals of the deposit weights (amplitude-level, not timing).
 struct RcpRn {
     __device__ __forceinline__ double operator()(double x) const { return rcp_rn(x); }
 };
-
-// LDS window accumulator: cell (bin j, component c, lane) at ((j-b0)*6+c)*64+lane.
-// Each cell has exactly one writer (its own lane), and a wave's LDS accesses
-// execute in order, so a plain read-add-write is race-free and deterministic
-// (LDS float atomics serialise per lane and are much slower here).
+struct RcpF32 {
+    __device__ __forceinline__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
+};
 template <class F>
-struct LdsAcc {
-    F* base;  // lds window + lane
+struct RcpOf;
+template <>
+struct RcpOf<double> { using type = RcpRn; };
+template <>
+struct RcpOf<float> { using type = RcpF32; };
+
+// LDS trace window of one wave: cell (bucket j, component c, lane) at
+// ((j*6)+c)*64 + lane, W buckets plus two per-lane trash buckets (W, W+1)
+// that absorb the absent targets of the branch-free common deposit.  Every
+// cell has exactly one writer (its own lane) and a wave's LDS accesses execute
+// in order, so plain read-add-write is race-free and deterministic.
+template <class F>
+struct Window {
+    F* lane_base;  // lds + lane
+    __device__ __forceinline__ F* cell(int j) const { return lane_base + j * (6 * kLanes); }
+    // bucket ja += aa f1 + ba f2 and bucket jb += ab f1 + bb f2 (ja != jb)
+    __device__ __forceinline__ void add2(int ja, F aa, F ba, int jb, F ab, F bb, const F* f1, const F* f2) const
+    {
+        F* ca = cell(ja);
+        F* cb = cell(jb);
+        F va[6], vb[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) va[c] = ca[c * kLanes];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vb[c] = cb[c * kLanes];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ca[c * kLanes] = va[c] + (aa * f1[c] + ba * f2[c]);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cb[c * kLanes] = vb[c] + (ab * f1[c] + bb * f2[c]);
+    }
+};
+
+// Accumulator for step_general: absolute bucket j -> window cell j - b0.
+template <class F>
+struct WindowAcc {
+    Window<F> w;
     int b0;
     const F* f1;
     const F* f2;
     __device__ __forceinline__ void operator()(int j, double a, double b) const
     {
-        F* cell = base + (j - b0) * (6 * kLanes);
+        F* c0 = w.cell(j - b0);
         const F fa = (F)a, fb = (F)b;
-        F v[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v[c] = cell[c * kLanes];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) cell[c * kLanes] = v[c] + (fa * f1[c] + fb * f2[c]);
+        for (int c = 0; c < 6; ++c) c0[c * kLanes] += fa * f1[c] + fb * f2[c];
     }
 };
 
@@ -261,97 +287,55 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
-// Trajectory records of the particle being walked are staged through LDS in
-// chunks of 16 (1536 B): each lane loads 24 B of a chunk with ordinary
-// coalesced loads one chunk ahead of use, and the wave reads the record of the
-// current step from LDS (same address for all lanes).  The wait for a chunk
-// therefore falls once per 16 steps instead of once per step.
-constexpr int kChunkRecs = 16;
-constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
-
-template <class Rec>
-struct Stager {
-    const double* __restrict__ base;  // the particle's records, as doubles
-    double* slots;                    // LDS: 2 chunks
-    int lane;
-    int lim;                          // last legal 3-double group start (12*NP - 3)
-    int next_fill;                    // chunk held in (q0,q1,q2)
-    double q0, q1, q2;
-
-    __device__ __forceinline__ void load(int chunk, double& a, double& b, double& c) const
-    {
-        int i = kChunkDoubles * chunk + 3 * lane;
-        i = i < lim ? i : lim;
-        a = base[i];
-        b = base[i + 1];
-        c = base[i + 2];
-    }
-    __device__ __forceinline__ void store(int chunk, double a, double b, double c) const
-    {
-        double* d = slots + (chunk & 1) * kChunkDoubles + 3 * lane;
-        d[0] = a;
-        d[1] = b;
-        d[2] = c;
-    }
-    // Stage the chunks holding records k and k+1 and request the next one.
-    __device__ __forceinline__ void start(int k)
-    {
-        const int c0 = k / kChunkRecs;
-        double a, b, c;
-        load(c0, a, b, c);
-        double d, e, f;
-        load(c0 + 1, d, e, f);
-        load(c0 + 2, q0, q1, q2);
-        store(c0, a, b, c);
-        store(c0 + 1, d, e, f);
-        next_fill = c0 + 2;
-    }
-    // Make records up to r available (r advances by at most 2 per call, so
-    // at most one chunk is crossed).
-    __device__ __forceinline__ void prepare(int r)
-    {
-        if (r / kChunkRecs == next_fill) {
-            store(next_fill, q0, q1, q2);
-            load(next_fill + 1, q0, q1, q2);
-            ++next_fill;
+// Deposit of step (p1 -> p2) into the window [b0, b1).
+template <class F>
+__device__ __forceinline__ void deposit_window(const Window<F>& win, int W, bool active, const PointT<F>& p1,
+                                               const PointT<F>& p2, double t0, double dt, double inv_dt,
+                                               double tmin, double tmax, int n_bins, int b0, int b1)
+{
+    const bool valid = active && p2.ts > tmin && p1.ts < tmax;    // particle.cpp:793
+    const bool case_a = p1.ts <= tmin;
+    const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
+    const bool gen = valid && step_is_general(case_a, p1.idx, i2);
+    const TwoBuckets<F> t = step_common<F>(typename RcpOf<F>::type{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2,
+                                           t0, dt, inv_dt, n_bins, b0, b1, W, W + 1);
+    win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
+    if (__ballot(gen)) {
+        if (gen) {
+            WindowAcc<F> acc{win, b0, p1.f, p2.f};
+            step_general(acc, rcp_rn(p2.ts - p1.ts), case_a, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt, n_bins,
+                         b0, b1);
         }
     }
-    __device__ __forceinline__ const Rec& at(int r) const
-    {
-        return *reinterpret_cast<const Rec*>(slots + ((r / kChunkRecs) & 1) * kChunkDoubles + (r % kChunkRecs) * 12);
-    }
-};
+}
 
 // Walk the steps of one particle starting at step k (uniform) for the window
-// [b0, b1).  Returns (through nk/nmin) the first step that reaches bins >= b1
-// for some lane and the smallest start bin of that step over the lanes.
-// Two trajectory points are evaluated per iteration as independent chains, so
-// a single wave keeps the SIMD busy while each chain waits on its latencies.
+// [b0, b1).  Returns (through nk/nmin) the first step that reaches buckets
+// >= b1 for some lane and the smallest start bucket of that step over the
+// lanes.  The trajectory record of the next point is requested by a scalar
+// load one step ahead of its use.  With verify, k is a searched guess: step
+// back until no lane starts at or beyond b0 at point k.
 template <int PREC>
 __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* __restrict__ rp, int k,
                                              int NP, bool active, double Px, double Py, double Pz,
                                              double t0, double dt, double inv_dt, double tmin,
-                                             double tmax, int n_bins, int b0, int b1,
-                                             typename RecOf<PREC>::amp* win_lane, double* stage, int lane,
-                                             bool verify, int& nk, int& nmin)
+                                             double tmax, int n_bins, int b0, int b1, int W,
+                                             const Window<typename RecOf<PREC>::amp>& win, bool verify,
+                                             int& nk, int& nmin)
 {
     using Rec = typename RecOf<PREC>::type;
     using F = typename RecOf<PREC>::amp;
-    Stager<Rec> sg{reinterpret_cast<const double*>(rp), stage, lane, 12 * NP - 3, 0, 0.0, 0.0, 0.0};
-    sg.start(k);
-    PointT<F> p0, q1, q2;
-    eval_point<PREC>(sg.at(k), Px, Py, Pz, t0, dt, inv_dt, p0);
+    PointT<F> p0, p1;
+    eval_point<PREC>(rp[k], Px, Py, Pz, t0, dt, inv_dt, p0);
     if (verify) {
-        // a searched cursor is a guess: step back until no lane starts inside
-        // or beyond the window at point k, so no earlier step reaches it
         int back = 32;
         while (k > 0 && __ballot(active && p0.idx >= b0)) {
             k = k > back ? k - back : 0;
             back *= 2;
-            sg.start(k);
-            eval_point<PREC>(sg.at(k), Px, Py, Pz, t0, dt, inv_dt, p0);
+            eval_point<PREC>(rp[k], Px, Py, Pz, t0, dt, inv_dt, p0);
         }
     }
+    Rec rn = rp[k + 1];
     int next_k = -1, next_min = INT_MAX;
     int steps = 0;
     while (true) {
@@ -363,46 +347,16 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
             }
             break;
         }
-        const int r2 = k + 2 < NP ? k + 2 : NP - 1;
-        sg.prepare(r2);
-        eval_point<PREC>(sg.at(k + 1), Px, Py, Pz, t0, dt, inv_dt, q1);
-        eval_point<PREC>(sg.at(r2), Px, Py, Pz, t0, dt, inv_dt, q2);
-        {
-            LdsAcc<F> acc{win_lane, b0, p0.f, q1.f};
-            deposit_step(acc, RcpRn{}, active, p0.ts, p0.idx, q1.ts, q1.idx, t0, dt, inv_dt, tmin, tmax, n_bins,
-                         b0, b1);
-        }
+        const Rec r = rn;
+        rn = rp[k + 2 < NP ? k + 2 : NP - 1];
+        eval_point<PREC>(r, Px, Py, Pz, t0, dt, inv_dt, p1);
+        deposit_window<F>(win, W, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
-        if (next_k < 0 && __ballot(active && q1.idx >= b1)) {
+        if (next_k < 0 && __ballot(active && p1.idx >= b1)) {
             next_k = k;
             next_min = wave_min(active ? p0.idx : INT_MAX);
         }
-        // second step (k+1, k+2) if it exists and some lane is still in the window
-        const bool two = (k + 1 < NP - 1) && __ballot(active && q1.idx < b1);
-        if (!two) {
-            ++k;
-            if (k >= NP - 1) {
-                if (next_k < 0) {
-                    next_k = NP - 1;
-                    next_min = INT_MAX;
-                }
-            } else if (next_k < 0) {
-                next_k = k;
-                next_min = wave_min(active ? q1.idx : INT_MAX);
-            }
-            break;
-        }
-        {
-            LdsAcc<F> acc{win_lane, b0, q1.f, q2.f};
-            deposit_step(acc, RcpRn{}, active, q1.ts, q1.idx, q2.ts, q2.idx, t0, dt, inv_dt, tmin, tmax, n_bins,
-                         b0, b1);
-        }
-        ++steps;
-        if (next_k < 0 && __ballot(active && q2.idx >= b1)) {
-            next_k = k + 1;
-            next_min = wave_min(active ? q1.idx : INT_MAX);
-        }
-        k += 2;
+        ++k;
         if (k >= NP - 1) {
             if (next_k < 0) {
                 next_k = NP - 1;
@@ -410,7 +364,7 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
             }
             break;
         }
-        p0 = q2;
+        p0 = p1;
     }
     nk = next_k;
     nmin = next_min;
@@ -425,7 +379,7 @@ __device__ __forceinline__ int search_first_point(const typename RecOf<PREC>::ty
                                                   double Px, double Py, double Pz, double t0, double dt,
                                                   double inv_dt, int b)
 {
-    int lo = 0, hi = NP - 1;   // invariant: answer in [lo, hi]; hi may not satisfy
+    int lo = 0, hi = NP - 1;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         const auto& r = rp_lane[mid];
@@ -445,8 +399,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                                                         const ScreenDev* __restrict__ screens,
                                                         const typename RecOf<PREC>::type* __restrict__ recs,
                                                         int n_points, int2* __restrict__ cursors,
-                                                        unsigned long long* __restrict__ stats,
-                                                        int lds_window)
+                                                        unsigned long long* __restrict__ stats)
 {
     using F = typename RecOf<PREC>::amp;
     constexpr int kFlushBatches = 16;   // mixed: FP32 window flushed every 16 x 64 particles
@@ -468,9 +421,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         tmax = t0 + dt * ((double)nb - 0.5);
     }
     int2* __restrict__ cur_base = cursors + it.cursor_off;
-    F* win_lane = reinterpret_cast<F*>(lds) + lane;
-    // 2 record chunks after the window (window bytes are a multiple of 1536)
-    double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)lds_window * 6 * kLanes);
+    const Window<F> win{reinterpret_cast<F*>(lds) + lane};
     unsigned long long n_steps = 0, n_walks = 0;
     // seed the cursors of a group that does not start at bucket 0
     if (it.bin_lo > 0) {
@@ -489,7 +440,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         const int b1 = b0 + W < it.bin_hi ? b0 + W : it.bin_hi;
         const int ncell = (b1 - b0) * 6;
         double* out = S.partial + (((size_t)it.chunk * S.n_pix + (active ? pix : 0)) * nb + b0) * 6;
-        for (int j = 0; j < ncell; ++j) win_lane[j * kLanes] = (F)0;
+        for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
         bool first_flush = true;
         int pending = 0;
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
@@ -506,9 +457,8 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                 const bool verify = __builtin_amdgcn_readlane(cur.y, j) == INT_MIN && k > 0;
                 const int p = pb + j;
                 int nk, nmin;
-                n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
-                                               inv_dt, tmin, tmax, nb, b0, b1, win_lane, stage, lane,
-                                               verify, nk, nmin);
+                n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
+                                               tmin, tmax, nb, b0, b1, W, win, verify, nk, nmin);
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
@@ -518,11 +468,11 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                 if (pending >= kFlushBatches) {
                     if (active) {
                         for (int j = 0; j < ncell; ++j) {
-                            double v = (double)win_lane[j * kLanes];
+                            double v = (double)win.lane_base[j * kLanes];
                             out[j] = first_flush ? v : out[j] + v;
                         }
                     }
-                    for (int j = 0; j < ncell; ++j) win_lane[j * kLanes] = (F)0;
+                    for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
                     first_flush = false;
                     pending = 0;
                 }
@@ -530,9 +480,9 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         }
         if (active) {
             if (PREC == 0 || first_flush) {
-                for (int j = 0; j < ncell; ++j) out[j] = (double)win_lane[j * kLanes];
+                for (int j = 0; j < ncell; ++j) out[j] = (double)win.lane_base[j * kLanes];
             } else if (pending > 0) {
-                for (int j = 0; j < ncell; ++j) out[j] += (double)win_lane[j * kLanes];
+                for (int j = 0; j < ncell; ++j) out[j] += (double)win.lane_base[j * kLanes];
             }
         }
     }
@@ -543,9 +493,9 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
 }
 
 template __global__ void radiation_kernel<0>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
-                                             unsigned long long*, int);
+                                             unsigned long long*);
 template __global__ void radiation_kernel<1>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
-                                             unsigned long long*, int);
+                                             unsigned long long*);
 
 // ---------------------------------------------------------------------------
 // Sum the chunk partials into the trace store, chunks in order.

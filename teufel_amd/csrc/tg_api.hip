@@ -62,6 +62,7 @@ struct tg_context {
     int n_particles = 0, n_points = 0;
     double* d_raw = nullptr;     // kNumRaw arrays of n_particles*n_points, then charge[n_particles]
     void* d_rec[2] = {nullptr, nullptr};
+    int* d_flag = nullptr;
     bool rec_valid[2] = {false, false};
     // screens
     std::vector<ScreenHost> screens;
@@ -169,7 +170,8 @@ int ensure_records(tg_context* c, int prec)
 {
     if (c->rec_valid[prec]) return TG_OK;
     size_t n = (size_t)c->n_particles * c->n_points;
-    if (!c->d_rec[prec]) TG_HIP(c, hipMalloc(&c->d_rec[prec], std::max<size_t>(n, 1) * 96));
+    if (!c->d_rec[prec])
+        TG_HIP(c, hipMalloc(&c->d_rec[prec], std::max<size_t>(n, 1) * (prec == TG_FP64 ? sizeof(RecF64) : sizeof(RecMixed))));
     if (n) {
         unsigned blocks = (unsigned)((n + 255) / 256);
         if (prec == TG_FP64)
@@ -178,10 +180,22 @@ int ensure_records(tg_context* c, int prec)
                                                          raw(c, 8), raw(c, 9), raw_charge(c),
                                                          (RecF64*)c->d_rec[prec]);
         else
+        {
+            if (!c->d_flag) TG_HIP(c, hipMalloc((void**)&c->d_flag, sizeof(int)));
+            TG_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
             pack_mixed_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
                                                            raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
                                                            raw(c, 8), raw(c, 9), raw_charge(c),
-                                                           (RecMixed*)c->d_rec[prec]);
+                                                           (RecMixed*)c->d_rec[prec], c->d_flag);
+            TG_HIP(c, hipGetLastError());
+            int off_axis = 0;
+            TG_HIP(c, hipMemcpyAsync(&off_axis, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            TG_HIP(c, hipStreamSynchronize(c->stream));
+            if (off_axis)
+                return fail(c, TG_ERR_ARG,
+                            "TG_MIXED needs beams travelling close to the z axis (|beta_x|+|beta_y| <= 0.25); "
+                            "use TG_FP64");
+        }
         TG_HIP(c, hipGetLastError());
     }
     c->rec_valid[prec] = true;
@@ -272,9 +286,10 @@ int default_window(const tg_context* c, int n_bins)
 
 size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
 
+// LDS of one wave: (window + 2 trash buckets) x 6 components x 64 lanes.
 int max_window_for_lds(const tg_context* c, int prec)
 {
-    return (int)((c->max_lds - 2 * kChunkDoubles * sizeof(double)) / (6 * kLanes * cell_bytes(prec)));
+    return (int)(c->max_lds / (6 * kLanes * cell_bytes(prec))) - 2;
 }
 
 // Build the work list for screens `ids`, particles [p0,p1).
@@ -389,16 +404,16 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = (size_t)plan.max_window * 6 * kLanes * cell_bytes(prec) + 2 * kChunkDoubles * sizeof(double);
+    size_t lds = (size_t)(plan.max_window + 2) * 6 * kLanes * cell_bytes(prec);
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (prec == TG_FP64)
         radiation_kernel<0><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
             c->d_items, c->d_sdev, (const RecF64*)c->d_rec[0], c->n_points, c->d_cursors,
-            c->stats_on ? c->d_stats : nullptr, plan.max_window);
+            c->stats_on ? c->d_stats : nullptr);
     else
         radiation_kernel<1><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
             c->d_items, c->d_sdev, (const RecMixed*)c->d_rec[1], c->n_points, c->d_cursors,
-            c->stats_on ? c->d_stats : nullptr, plan.max_window);
+            c->stats_on ? c->d_stats : nullptr);
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
@@ -577,6 +592,7 @@ int tg_destroy(tg_context* c)
     (void)hipFree(c->d_cursors);
     (void)hipFree(c->d_partial);
     (void)hipFree(c->d_stats);
+    (void)hipFree(c->d_flag);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -3,16 +3,22 @@
 // Restates the deposit of ChargedParticle::integrateFieldTrace
 // (src/particle.cpp:788-865): a step observed from ts1 to ts2 with end fields
 // f1, f2 is integrated, linearly interpolated, over trace buckets of width dt
-// centred at t0 + i dt, bucket index round((t - t0)/dt).  Three rules of the
-// reference cover every step:
+// centred at t0 + i dt, bucket index round((t - t0)/dt).  The reference's
+// rules:
 //   single   the step lies in one bucket:           (f1+f2)/2 * dts/dt
 //   partial  first / last partially covered bucket: mean of the interpolated
 //            end values times the covered time / dt
 //   full     fully covered bucket:                   f(bucket centre)
-// Every contribution is a*f1 + b*f2, added through acc(j, a, b).  Time values
-// (bucket edges and centres) are formed exactly as the reference forms them,
-// so only amplitude weights differ from the reference, by rounding; the
-// caller compiles this with FMA contraction off.
+// Every contribution is a*f1 + b*f2.  Time values (bucket edges and centres)
+// are formed exactly as the reference forms them, so only amplitude weights
+// differ from the reference, by rounding; callers compile this with FMA
+// contraction off.
+//
+// A step that neither starts before the trace nor spans three or more
+// buckets - nearly every step - touches one bucket or two adjacent ones split
+// at the edge e = t0 + (idx1 + 1/2) dt; step_common gives both targets and
+// weights without branches.  step_general covers the rest.  deposit_step
+// composes the two with the reference's semantics.
 //
 // TG_HD marks the functions for host and device; this header is included by
 // the HIP kernels and, with TG_HD empty, by a host test of the rule.
@@ -24,8 +30,47 @@
 
 namespace tg {
 
-// One bucket j of a step, any rule.  first: the step starts in bucket j (and
-// not before the trace); last: the step ends in bucket j (j < n_bins).
+// The two buckets of a common step: window-relative indices (the caller's
+// `none_a` / `none_b` when absent or outside [b0, b1)) and weights.
+template <class F>
+struct TwoBuckets {
+    int ja, jb;
+    F aa, ba, ab, bb;
+};
+
+// True when the step needs step_general: it starts before the trace
+// (ts1 <= tmin) or spans three or more buckets.
+TG_HD inline bool step_is_general(bool case_a, int idx1, int i2) { return case_a || i2 > idx1 + 1; }
+
+// Common step (valid, not general) in weight type F; rcp(x) = 1/x in F.
+//   bucket A = idx1: h_A (f1 + f_e),  bucket B = idx1+1: h_B (f_e + f2),
+//   f_e = f1 (ts2-e)/dts + f2 (e-ts1)/dts,  h_A = (e-ts1)/(2dt), h_B = (ts2-e)/(2dt);
+// a single-bucket step is the case e = ts2.
+template <class F, class Rcp>
+TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, int idx1, double ts2, int i2,
+                                       double t0, double dt, double inv_dt, int n_bins, int b0, int b1,
+                                       int none_a, int none_b)
+{
+#pragma clang fp contract(off)
+    TwoBuckets<F> o;
+    const bool single = i2 == idx1;
+    const double e = single ? ts2 : t0 + (idx1 + 0.5) * dt;     // particle.cpp:841 t_end
+    const F da = (F)(e - ts1), db = (F)(ts2 - e);               // exact time differences
+    const F idts = rcp(da + db);
+    const F la = da * idts, lb = db * idts;
+    const F fid = (F)inv_dt;
+    const F ha = (F)0.5 * (da * fid), hb = (F)0.5 * (db * fid);
+    o.aa = ha * ((F)1 + lb);
+    o.ba = ha * la;
+    o.ab = hb * lb;
+    o.bb = hb * (la + (F)1);
+    o.ja = (common && idx1 >= b0 && idx1 < b1) ? idx1 - b0 : none_a;
+    o.jb = (common && !single && i2 < n_bins && i2 >= b0 && i2 < b1) ? i2 - b0 : none_b;
+    return o;
+}
+
+// One bucket j of a general step.  first: the step starts in bucket j (and not
+// before the trace); last: the step ends in bucket j (j < n_bins).
 template <class Acc>
 TG_HD inline void deposit_bucket(const Acc& acc, int j, bool first, bool last, double ts1, double ts2,
                                  double idts, double t0, double dt, double inv_dt)
@@ -42,41 +87,38 @@ TG_HD inline void deposit_bucket(const Acc& acc, int j, bool first, bool last, d
     acc(j, h * ((ts2 - left) * idts + (ts2 - right) * idts), h * ((left - ts1) * idts + (right - ts1) * idts));
 }
 
-// The whole step, restricted to buckets [b0, b1).  idx1/idx2 are the bucket
-// indices of ts1/ts2 (round half away from zero); tmin/tmax the trace span
-// t0 - dt/2, t0 + (n_bins - 1/2) dt; rcp(x) returns 1/x.
+// General step (valid): every bucket of its range within [b0, b1).
+template <class Acc>
+TG_HD inline void step_general(const Acc& acc, double idts, bool case_a, double ts1, int idx1, double ts2, int i2,
+                               double t0, double dt, double inv_dt, int n_bins, int b0, int b1)
+{
+    const int lo = case_a ? 0 : idx1;
+    const int hi = i2 < n_bins - 1 ? i2 : n_bins - 1;
+    const int jlo = lo > b0 ? lo : b0;
+    const int jhi = hi < b1 - 1 ? hi : b1 - 1;
+    for (int j = jlo; j <= jhi; ++j)
+        deposit_bucket(acc, j, !case_a && j == idx1, j == i2, ts1, ts2, idts, t0, dt, inv_dt);
+}
+
+// The whole step with the reference's semantics, restricted to buckets
+// [b0, b1); acc(j, a, b) adds a*f1 + b*f2 into bucket j.  idx1/idx2 are the
+// bucket indices of ts1/ts2, tmin/tmax the trace span.
 template <class Acc, class Rcp>
 TG_HD inline void deposit_step(const Acc& acc, const Rcp& rcp, bool active, double ts1, int idx1, double ts2,
                                int idx2, double t0, double dt, double inv_dt, double tmin, double tmax,
                                int n_bins, int b0, int b1)
 {
-#pragma clang fp contract(off)
     if (!(active && ts2 > tmin && ts1 < tmax)) return;          // particle.cpp:793
-    const bool caseA = ts1 <= tmin;                             // starts before the trace
-    const int i2 = idx2 > n_bins ? n_bins : idx2;
-    const double dts = ts2 - ts1;
-    if (!caseA && idx1 == i2) {
-        // single bucket (particle.cpp:835-839)
-        if (idx1 >= b0 && idx1 < b1) {
-            const double w = 0.5 * (dts * inv_dt);
-            acc(idx1, w, w);
-        }
+    const bool case_a = ts1 <= tmin;                            // particle.cpp:796
+    const int i2 = idx2 > n_bins ? n_bins : idx2;               // particle.cpp:806,834
+    if (step_is_general(case_a, idx1, i2)) {
+        step_general(acc, rcp(ts2 - ts1), case_a, ts1, idx1, ts2, i2, t0, dt, inv_dt, n_bins, b0, b1);
         return;
     }
-    const double idts = rcp(dts);
-    if (!caseA && i2 == idx1 + 1) {
-        // two buckets split at the edge t_end = t0 + (idx1 + 1/2) dt
-        if (idx1 >= b0 && idx1 < b1) deposit_bucket(acc, idx1, true, false, ts1, ts2, idts, t0, dt, inv_dt);
-        if (i2 < n_bins && i2 >= b0 && i2 < b1) deposit_bucket(acc, i2, false, true, ts1, ts2, idts, t0, dt, inv_dt);
-        return;
-    }
-    // general: a step over three or more buckets, or one that starts before the trace
-    const int lo = caseA ? 0 : idx1;
-    const int hi = i2 < n_bins - 1 ? i2 : n_bins - 1;
-    const int jlo = lo > b0 ? lo : b0;
-    const int jhi = hi < b1 - 1 ? hi : b1 - 1;
-    for (int j = jlo; j <= jhi; ++j)
-        deposit_bucket(acc, j, !caseA && j == idx1, j == i2, ts1, ts2, idts, t0, dt, inv_dt);
+    const TwoBuckets<double> t =
+        step_common<double>(rcp, true, ts1, idx1, ts2, i2, t0, dt, inv_dt, n_bins, b0, b1, -1, -1);
+    if (t.ja >= 0) acc(t.ja + b0, t.aa, t.ba);
+    if (t.jb >= 0) acc(t.jb + b0, t.ab, t.bb);
 }
 
 }  // namespace tg

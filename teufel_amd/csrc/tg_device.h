@@ -40,19 +40,21 @@ struct alignas(32) RecF64 {
     double scale;   // q e / (4 pi eps0)
 };
 
-// Packed trajectory point for the mixed kernel (96 B): FP64 where the
-// retarded time and 1-n.beta need it, FP32 amplitude inputs with the charge
-// scale folded in.
-struct alignas(32) RecMixed {
+// Packed trajectory point for the mixed kernel: 64 B, one scalar load.  FP64
+// where the retarded time and the dominant term of 1 - n.beta need it (t, X,
+// beta_z); FP32 for the transverse beta, beta' pre-multiplied by the charge
+// scale / c, and the velocity-term scale / gamma^2.  The FP32 transverse
+// beta assumes the beam travels close to the z axis (|beta_x|+|beta_y| small),
+// as in TEUFEL's undulator lattices; the pack kernel flags other beams.
+struct alignas(64) RecMixed {
     double t, x, y, z;
-    double bx, by, bz;
-    float bpx, bpy, bpz;   // beta'
-    float sv;              // scale / gamma^2      (velocity term)
-    float sa;              // scale / c            (acceleration term)
-    float fbx, fby, fbz;   // beta in FP32
+    double bz;
+    float bx, by;
+    float apx, apy, apz;   // beta' * scale / c
+    float sv;              // scale / gamma^2
 };
 static_assert(sizeof(RecF64) == 96, "RecF64 layout");
-static_assert(sizeof(RecMixed) == 96, "RecMixed layout");
+static_assert(sizeof(RecMixed) == 64, "RecMixed layout");
 
 // One screen as the kernel sees it.
 struct ScreenDev {
