@@ -309,6 +309,104 @@ __device__ __forceinline__ void deposit_window(const Window<F>& win, int W, bool
     }
 }
 
+// Two ways to feed the wave the (uniform) trajectory record of each step.
+//
+// LdsStager (FP64 kernel, 96-B records): the particle's records are staged
+// through LDS in chunks of 16 (1536 B); each lane loads 24 B of a chunk with
+// coalesced vector loads one chunk ahead of use, and the wave reads the
+// current record from LDS.  The wait for a chunk falls once per 16 steps.
+//
+// ScalarPrefetch (mixed kernel, 64-B records): one scalar load per record,
+// requested one step ahead of use.
+constexpr int kChunkRecs = 16;
+constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
+
+template <class Rec>
+struct LdsStager {
+    const Rec* __restrict__ rp;
+    double* slots;                    // LDS: 2 chunks
+    int lane;
+    int lim;                          // last legal 3-double group start (12*NP - 3)
+    int next_fill;                    // chunk held in (q0,q1,q2)
+    double q0, q1, q2;
+
+    __device__ __forceinline__ void load(int chunk, double& a, double& b, double& c) const
+    {
+        const double* base = reinterpret_cast<const double*>(rp);
+        int i = kChunkDoubles * chunk + 3 * lane;
+        i = i < lim ? i : lim;
+        a = base[i];
+        b = base[i + 1];
+        c = base[i + 2];
+    }
+    __device__ __forceinline__ void store(int chunk, double a, double b, double c) const
+    {
+        double* d = slots + (chunk & 1) * kChunkDoubles + 3 * lane;
+        d[0] = a;
+        d[1] = b;
+        d[2] = c;
+    }
+    // Stage the chunks holding records k and k+1 and request the next one.
+    __device__ __forceinline__ void start(int k)
+    {
+        const int c0 = k / kChunkRecs;
+        double a, b, c;
+        load(c0, a, b, c);
+        double d, e, f;
+        load(c0 + 1, d, e, f);
+        load(c0 + 2, q0, q1, q2);
+        store(c0, a, b, c);
+        store(c0 + 1, d, e, f);
+        next_fill = c0 + 2;
+    }
+    // Record r, r advancing by one per call after start.
+    __device__ __forceinline__ const Rec& next(int r)
+    {
+        if (r / kChunkRecs == next_fill) {
+            store(next_fill, q0, q1, q2);
+            load(next_fill + 1, q0, q1, q2);
+            ++next_fill;
+        }
+        return *reinterpret_cast<const Rec*>(slots + ((r / kChunkRecs) & 1) * kChunkDoubles + (r % kChunkRecs) * 12);
+    }
+    __device__ __forceinline__ const Rec& first(int k) const
+    {
+        return *reinterpret_cast<const Rec*>(slots + ((k / kChunkRecs) & 1) * kChunkDoubles + (k % kChunkRecs) * 12);
+    }
+};
+
+template <class Rec>
+struct ScalarPrefetch {
+    const Rec* __restrict__ rp;
+    int NP;
+    Rec rn;   // record of the next point
+    __device__ __forceinline__ void start(int k) { rn = rp[k + 1]; }
+    __device__ __forceinline__ Rec first(int k) const { return rp[k]; }
+    __device__ __forceinline__ Rec next(int r)
+    {
+        const Rec cur = rn;
+        rn = rp[r + 1 < NP ? r + 1 : NP - 1];
+        return cur;
+    }
+};
+
+template <int PREC>
+struct SourceOf;
+template <>
+struct SourceOf<0> { using type = LdsStager<RecF64>; static constexpr int lds_doubles = 2 * kChunkDoubles; };
+template <>
+struct SourceOf<1> { using type = ScalarPrefetch<RecMixed>; static constexpr int lds_doubles = 0; };
+
+template <int PREC>
+__device__ __forceinline__ typename SourceOf<PREC>::type make_source(const typename RecOf<PREC>::type* rp, int NP,
+                                                                     double* stage, int lane)
+{
+    if constexpr (PREC == 0)
+        return LdsStager<RecF64>{rp, stage, lane, 12 * NP - 3, 0, 0.0, 0.0, 0.0};
+    else
+        return ScalarPrefetch<RecMixed>{rp, NP, RecMixed{}};
+}
+
 // Walk the steps of one particle starting at step k (uniform) for the window
 // [b0, b1).  Returns (through nk/nmin) the first step that reaches buckets
 // >= b1 for some lane and the smallest start bucket of that step over the
@@ -320,22 +418,23 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
                                              int NP, bool active, double Px, double Py, double Pz,
                                              double t0, double dt, double inv_dt, double tmin,
                                              double tmax, int n_bins, int b0, int b1, int W,
-                                             const Window<typename RecOf<PREC>::amp>& win, bool verify,
-                                             int& nk, int& nmin)
+                                             const Window<typename RecOf<PREC>::amp>& win, double* stage,
+                                             int lane, bool verify, int& nk, int& nmin)
 {
-    using Rec = typename RecOf<PREC>::type;
     using F = typename RecOf<PREC>::amp;
+    auto src = make_source<PREC>(rp, NP, stage, lane);
     PointT<F> p0, p1;
-    eval_point<PREC>(rp[k], Px, Py, Pz, t0, dt, inv_dt, p0);
+    src.start(k);
+    eval_point<PREC>(src.first(k), Px, Py, Pz, t0, dt, inv_dt, p0);
     if (verify) {
         int back = 32;
         while (k > 0 && __ballot(active && p0.idx >= b0)) {
             k = k > back ? k - back : 0;
             back *= 2;
-            eval_point<PREC>(rp[k], Px, Py, Pz, t0, dt, inv_dt, p0);
+            src.start(k);
+            eval_point<PREC>(src.first(k), Px, Py, Pz, t0, dt, inv_dt, p0);
         }
     }
-    Rec rn = rp[k + 1];
     int next_k = -1, next_min = INT_MAX;
     int steps = 0;
     while (true) {
@@ -347,9 +446,7 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
             }
             break;
         }
-        const Rec r = rn;
-        rn = rp[k + 2 < NP ? k + 2 : NP - 1];
-        eval_point<PREC>(r, Px, Py, Pz, t0, dt, inv_dt, p1);
+        eval_point<PREC>(src.next(k + 1), Px, Py, Pz, t0, dt, inv_dt, p1);
         deposit_window<F>(win, W, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
         if (next_k < 0 && __ballot(active && p1.idx >= b1)) {
@@ -399,7 +496,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                                                         const ScreenDev* __restrict__ screens,
                                                         const typename RecOf<PREC>::type* __restrict__ recs,
                                                         int n_points, int2* __restrict__ cursors,
-                                                        unsigned long long* __restrict__ stats)
+                                                        unsigned long long* __restrict__ stats, int lds_window)
 {
     using F = typename RecOf<PREC>::amp;
     constexpr int kFlushBatches = 16;   // mixed: FP32 window flushed every 16 x 64 particles
@@ -422,6 +519,8 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
     }
     int2* __restrict__ cur_base = cursors + it.cursor_off;
     const Window<F> win{reinterpret_cast<F*>(lds) + lane};
+    // FP64: two record chunks after the window and its trash buckets
+    double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)(lds_window + 2) * 6 * kLanes);
     unsigned long long n_steps = 0, n_walks = 0;
     // seed the cursors of a group that does not start at bucket 0
     if (it.bin_lo > 0) {
@@ -458,7 +557,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                 const int p = pb + j;
                 int nk, nmin;
                 n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
-                                               tmin, tmax, nb, b0, b1, W, win, verify, nk, nmin);
+                                               tmin, tmax, nb, b0, b1, W, win, stage, lane, verify, nk, nmin);
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
@@ -493,9 +592,9 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
 }
 
 template __global__ void radiation_kernel<0>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
-                                             unsigned long long*);
+                                             unsigned long long*, int);
 template __global__ void radiation_kernel<1>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
-                                             unsigned long long*);
+                                             unsigned long long*, int);
 
 // ---------------------------------------------------------------------------
 // Sum the chunk partials into the trace store, chunks in order.

@@ -286,10 +286,13 @@ int default_window(const tg_context* c, int n_bins)
 
 size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
 
-// LDS of one wave: (window + 2 trash buckets) x 6 components x 64 lanes.
+// LDS of one wave: (window + 2 trash buckets) x 6 components x 64 lanes, plus
+// the FP64 kernel's record chunks.
+size_t stage_bytes(int prec) { return (prec == TG_FP64 ? SourceOf<0>::lds_doubles : SourceOf<1>::lds_doubles) * sizeof(double); }
+
 int max_window_for_lds(const tg_context* c, int prec)
 {
-    return (int)(c->max_lds / (6 * kLanes * cell_bytes(prec))) - 2;
+    return (int)((c->max_lds - stage_bytes(prec)) / (6 * kLanes * cell_bytes(prec))) - 2;
 }
 
 // Build the work list for screens `ids`, particles [p0,p1).
@@ -404,16 +407,16 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = (size_t)(plan.max_window + 2) * 6 * kLanes * cell_bytes(prec);
+    size_t lds = (size_t)(plan.max_window + 2) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec);
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (prec == TG_FP64)
         radiation_kernel<0><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
             c->d_items, c->d_sdev, (const RecF64*)c->d_rec[0], c->n_points, c->d_cursors,
-            c->stats_on ? c->d_stats : nullptr);
+            c->stats_on ? c->d_stats : nullptr, plan.max_window);
     else
         radiation_kernel<1><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
             c->d_items, c->d_sdev, (const RecMixed*)c->d_rec[1], c->n_points, c->d_cursors,
-            c->stats_on ? c->d_stats : nullptr);
+            c->stats_on ? c->d_stats : nullptr, plan.max_window);
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
