@@ -60,6 +60,7 @@ struct tg_context {
     size_t max_lds = 65536;
     // trajectories
     int n_particles = 0, n_points = 0;
+    bool traj_set = false;
     double* d_raw = nullptr;     // kNumRaw arrays of n_particles*n_points, then charge[n_particles]
     void* d_rec[2] = {nullptr, nullptr};
     int* d_flag = nullptr;
@@ -160,9 +161,10 @@ int alloc_traj(tg_context* c, int n_particles, int n_points)
     int rc = free_traj(c);
     if (rc) return rc;
     size_t n = (size_t)n_particles * (size_t)n_points;
-    TG_HIP(c, hipMalloc((void**)&c->d_raw, (kNumRaw * n + (size_t)n_particles) * sizeof(double)));
+    TG_HIP(c, hipMalloc((void**)&c->d_raw, std::max<size_t>(1, kNumRaw * n + (size_t)n_particles) * sizeof(double)));
     c->n_particles = n_particles;
     c->n_points = n_points;
+    c->traj_set = true;
     return TG_OK;
 }
 
@@ -435,8 +437,7 @@ int check_ctx(const tg_context* c) { return c ? TG_OK : TG_ERR_ARG; }
 int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id, int p0, int p1)
 {
     if (prec != TG_FP64 && prec != TG_MIXED) return fail(c, TG_ERR_ARG, "precision must be TG_FP64 or TG_MIXED");
-    if (!c->d_raw && c->n_particles == 0 && !c->screens.empty())
-        return fail(c, TG_ERR_STATE, "tg_integrate: no trajectories set");
+    if (!c->traj_set && !c->screens.empty()) return fail(c, TG_ERR_STATE, "tg_integrate: no trajectories set");
     if (screen_id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
     if (p0 < 0 || p1 > c->n_particles || p0 > p1) return fail(c, TG_ERR_RANGE, "particle range out of range");
     c->kernel_ms = c->total_ms = 0.f;

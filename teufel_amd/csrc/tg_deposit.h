@@ -56,8 +56,10 @@ TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, 
     const bool single = i2 == idx1;
     const double e = single ? ts2 : t0 + (idx1 + 0.5) * dt;     // particle.cpp:841 t_end
     const F da = (F)(e - ts1), db = (F)(ts2 - e);               // exact time differences
-    const F idts = rcp(da + db);
-    const F la = da * idts, lb = db * idts;
+    // a single-bucket step has la = 1, lb = 0 exactly (also for a zero-length
+    // step, which adds exact zeros as in the reference)
+    const F idts = rcp(single ? (F)1 : da + db);
+    const F la = single ? (F)1 : da * idts, lb = single ? (F)0 : db * idts;
     const F fid = (F)inv_dt;
     const F ha = (F)0.5 * (da * fid), hb = (F)0.5 * (db * fid);
     o.aa = ha * ((F)1 + lb);

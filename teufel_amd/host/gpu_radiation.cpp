@@ -1,0 +1,189 @@
+// gpu_radiation.cpp - host adapter over the C ABI.
+#include "gpu_radiation.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+
+#include "../../include/teufel_gpu.h"
+
+namespace teufel_gpu {
+namespace {
+
+struct State {
+    std::mutex mu;
+    tg_context* ctx = nullptr;
+    int device = -1;
+    int precision = -1;
+    // trajectory cache: (particle, version) list of the last upload
+    std::vector<std::pair<const ChargedParticle*, unsigned long long>> key;
+    ~State()
+    {
+        if (ctx) tg_destroy(ctx);
+    }
+};
+
+State& state()
+{
+    static State s;
+    return s;
+}
+
+void check(tg_context* ctx, int rc, const char* what)
+{
+    if (rc != TG_OK) {
+        std::string m = std::string("teufel_gpu: ") + what + " failed (status " + std::to_string(rc) + ")";
+        if (ctx) m += ": " + std::string(tg_last_error(ctx));
+        throw IOexception(m);
+    }
+}
+
+tg_context* context(State& s)
+{
+    if (!s.ctx) {
+        if (s.device < 0) {
+            const char* d = std::getenv("TEUFEL_GPU_DEVICE");
+            s.device = d ? std::atoi(d) : 0;
+        }
+        if (s.precision < 0) {
+            const char* p = std::getenv("TEUFEL_GPU_PRECISION");
+            s.precision = (p && std::strcmp(p, "mixed") == 0) ? TG_MIXED : TG_FP64;
+        }
+        check(nullptr, tg_create(s.device, &s.ctx), "tg_create");
+    }
+    return s.ctx;
+}
+
+void upload(State& s, const std::vector<ChargedParticle*>& ps)
+{
+    std::vector<std::pair<const ChargedParticle*, unsigned long long>> key;
+    key.reserve(ps.size());
+    for (const ChargedParticle* p : ps) key.emplace_back(p, p->version());
+    if (key == s.key) return;
+    int np = (int)ps.size(), npts = 0;
+    for (const ChargedParticle* p : ps) npts = std::max(npts, p->getNP());
+    size_t n = (size_t)np * npts;
+    std::vector<double> a(10 * n), q(np);
+    double* t = a.data();
+    double *x = t + n, *y = x + n, *z = y + n, *ux = z + n, *uy = ux + n, *uz = uy + n, *ax = uz + n, *ay = ax + n,
+           *az = ay + n;
+    for (int i = 0; i < np; i++) {
+        const ChargedParticle* p = ps[i];
+        q[i] = p->getCharge();
+        for (int k = 0; k < npts; k++) {
+            int kk = p->getNP() > 0 ? std::min(k, p->getNP() - 1) : -1;
+            size_t g = (size_t)i * npts + k;
+            if (kk < 0) {
+                t[g] = x[g] = y[g] = z[g] = ux[g] = uy[g] = uz[g] = ax[g] = ay[g] = az[g] = 0.0;
+                continue;
+            }
+            Vector X = p->TrajPoint(kk), P = p->TrajMomentum(kk), A = p->TrajAccel(kk);
+            t[g] = p->TrajTime(kk);
+            x[g] = X.x; y[g] = X.y; z[g] = X.z;
+            ux[g] = P.x; uy[g] = P.y; uz[g] = P.z;
+            ax[g] = A.x; ay[g] = A.y; az[g] = A.z;
+        }
+    }
+    check(s.ctx, tg_set_trajectories(s.ctx, np, npts, t, x, y, z, ux, uy, uz, ax, ay, az, q.data()),
+          "tg_set_trajectories");
+    s.key.swap(key);
+}
+
+}  // namespace
+
+void set_precision(int p)
+{
+    if (p != TG_FP64 && p != TG_MIXED) throw IOexception("teufel_gpu::set_precision - TG_FP64 or TG_MIXED");
+    std::lock_guard<std::mutex> g(state().mu);
+    state().precision = p;
+}
+
+int precision()
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    context(s);
+    return s.precision;
+}
+
+void set_device(int device)
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (s.ctx && device != s.device) {
+        tg_destroy(s.ctx);
+        s.ctx = nullptr;
+        s.key.clear();
+    }
+    s.device = device;
+}
+
+void integrate(const std::vector<ChargedParticle*>& particles, const std::vector<Vector>& points,
+               const std::vector<FieldTrace*>& traces)
+{
+    if (points.size() != traces.size()) throw IOexception("teufel_gpu::integrate - points/traces size mismatch.");
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    tg_context* ctx = context(s);
+    upload(s, particles);
+    check(ctx, tg_remove_screens(ctx), "tg_remove_screens");
+    // one screen per (dt, N) class of traces
+    std::map<std::pair<double, size_t>, std::vector<size_t>> classes;
+    for (size_t i = 0; i < traces.size(); i++) {
+        if (!traces[i]) throw IOexception("teufel_gpu::integrate - null FieldTrace.");
+        classes[std::make_pair(traces[i]->get_dt(), traces[i]->get_N())].push_back(i);
+    }
+    std::vector<std::pair<int, const std::vector<size_t>*>> screens;
+    for (const auto& c : classes) {
+        const std::vector<size_t>& idx = c.second;
+        std::vector<double> obs(3 * idx.size()), t0(idx.size());
+        for (size_t j = 0; j < idx.size(); j++) {
+            const Vector& P = points[idx[j]];
+            obs[3 * j] = P.x; obs[3 * j + 1] = P.y; obs[3 * j + 2] = P.z;
+            t0[j] = traces[idx[j]]->get_t0();
+        }
+        int sid = -1;
+        check(ctx, tg_add_screen(ctx, (int)idx.size(), obs.data(), t0.data(), c.first.first, (int)c.first.second, &sid),
+              "tg_add_screen");
+        screens.emplace_back(sid, &c.second);
+    }
+    check(ctx, tg_integrate(ctx, s.precision, nullptr), "tg_integrate");
+    for (const auto& sc : screens) {
+        const std::vector<size_t>& idx = *sc.second;
+        size_t nb = traces[idx[0]]->get_N();
+        std::vector<double> buf(idx.size() * nb * 6);
+        check(ctx, tg_read_traces(ctx, sc.first, buf.data(), 0), "tg_read_traces");
+        for (size_t j = 0; j < idx.size(); j++) {
+            double* d = traces[idx[j]]->data();
+            const double* src = &buf[j * nb * 6];
+            for (size_t k = 0; k < nb * 6; k++) d[k] += src[k];
+        }
+    }
+    check(ctx, tg_remove_screens(ctx), "tg_remove_screens");
+}
+
+void deposit_only(int n_pseudo, int n_points, const double* ts, const double* fields, FieldTrace* trace)
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    tg_context* ctx = context(s);
+    check(ctx,
+          tg_deposit_only(ctx, n_pseudo, n_points, ts, fields, trace->get_t0(), trace->get_dt(), (int)trace->get_N(),
+                          trace->data()),
+          "tg_deposit_only");
+}
+
+void shutdown()
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    if (s.ctx) tg_destroy(s.ctx);
+    s.ctx = nullptr;
+    s.key.clear();
+}
+
+}  // namespace teufel_gpu

@@ -1,0 +1,40 @@
+// gpu_radiation.h - adapter from the host classes to the C ABI
+// (include/teufel_gpu.h).  One device context per process, created on first
+// use; calls are serialised by a mutex (the reference may call
+// integrateFieldTrace from OpenMP threads, src/mesh.cpp:296-301; the efficient
+// path is one Observer::integrate() for all pixels).  Non-zero status codes
+// of the C ABI become IOexception.
+#pragma once
+
+#include <vector>
+
+#include "fieldtrace.h"
+#include "particle.h"
+
+namespace teufel_gpu {
+
+// TG_FP64 (0, default) or TG_MIXED (1); the environment variable
+// TEUFEL_GPU_PRECISION=mixed selects mixed at first use.
+void set_precision(int precision);
+int precision();
+
+// Device ordinal used when the context is created (default 0, or the
+// environment variable TEUFEL_GPU_DEVICE).
+void set_device(int device);
+
+// Accumulate the fields of `particles` (summed in this order) at `points`
+// into `traces` (one per point; each has its own t0, and traces sharing
+// dt and N are integrated in one batched launch).  Particles with fewer
+// trajectory points are padded by repeating their last point, which adds
+// exact zeros, as the reference's step loop would.
+void integrate(const std::vector<ChargedParticle*>& particles, const std::vector<Vector>& points,
+               const std::vector<FieldTrace*>& traces);
+
+// The device bucket deposit driven by injected retarded times ts[n_pseudo][n_points]
+// and fields fields[n_pseudo][n_points][6], accumulated into trace.
+void deposit_only(int n_pseudo, int n_points, const double* ts, const double* fields, FieldTrace* trace);
+
+// Release the device context (also done at process exit).
+void shutdown();
+
+}  // namespace teufel_gpu

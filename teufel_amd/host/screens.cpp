@@ -1,0 +1,227 @@
+// screens.cpp - MeshedScreen, ScreenObserver and PointObserver for Beam and Bunch.
+#include "screens.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "gpu_radiation.h"
+
+void writeDump(const std::string& filename, const std::vector<Vector>& points, const std::vector<FieldTrace*>& traces)
+{
+    if (filename.empty()) return;
+    FILE* f = std::fopen(filename.c_str(), "wb");
+    if (!f) throw IOexception("writeDump - cannot open " + filename);
+    const char magic[8] = {'T', 'G', 'D', 'U', 'M', 'P', '1', 0};
+    int32_t np = (int32_t)points.size(), ns = traces.empty() ? 0 : (int32_t)traces[0]->get_N();
+    double dt = traces.empty() ? 0.0 : traces[0]->get_dt();
+    bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(&np, 4, 1, f) == 1 && std::fwrite(&ns, 4, 1, f) == 1 &&
+              std::fwrite(&dt, 8, 1, f) == 1;
+    for (size_t i = 0; ok && i < points.size(); i++) {
+        double head[4] = {points[i].x, points[i].y, points[i].z, traces[i]->get_t0()};
+        ok = std::fwrite(head, 8, 4, f) == 4 &&
+             std::fwrite(traces[i]->data(), 8, 6 * traces[i]->get_N(), f) == 6 * traces[i]->get_N();
+    }
+    std::fclose(f);
+    if (!ok) throw IOexception("writeDump - write error on " + filename);
+}
+
+namespace {
+double* traces_to_buffer(const std::vector<FieldTrace*>& tr, std::size_t n)
+{
+    double* buffer = new double[n];
+    double* bp = buffer;
+    for (const FieldTrace* t : tr) {
+        std::memcpy(bp, t->data(), 6 * t->get_N() * sizeof(double));
+        bp += 6 * t->get_N();
+    }
+    return buffer;
+}
+
+void buffer_to_traces(const double* buffer, std::size_t size, std::size_t expect, const std::vector<FieldTrace*>& tr,
+                      const char* who)
+{
+    if (size != expect) throw IOexception(std::string(who) + "::fromBuffer() - buffer size mismatch.");
+    const double* bp = buffer;
+    for (FieldTrace* t : tr) {
+        std::memcpy(t->data(), bp, 6 * t->get_N() * sizeof(double));
+        bp += 6 * t->get_N();
+    }
+}
+
+template <typename SourceT>
+void integrate_points(SourceT* source, const std::vector<Vector>& points, const std::vector<FieldTrace*>& traces)
+{
+    if (!source) throw RANGEexception("Observer::integrate() - no source set.");
+    std::vector<ChargedParticle*> ps;
+    collect_particles(source, ps);
+    teufel_gpu::integrate(ps, points, traces);
+}
+}  // namespace
+
+// ---- MeshedScreen ----------------------------------------------------------
+
+template <typename SourceT>
+MeshedScreen<SourceT>::MeshedScreen(const std::vector<Vector>& pts, const std::vector<double>& t0, double dt_,
+                                    int Nt_, const std::vector<double>& area_, const std::vector<Vector>& normal_,
+                                    const std::string& filename)
+    : Np((int)pts.size()), Nt(Nt_), dt(dt_), points(pts), area(area_), normal(normal_), FileName(filename)
+{
+    if (t0.size() != pts.size()) throw IOexception("MeshedScreen - points/t0 size mismatch.");
+    if (Nt < 0) throw IOexception("MeshedScreen - negative trace length.");
+    for (int i = 0; i < Np; i++) A.push_back(new FieldTrace(t0[i], dt, (std::size_t)Nt));
+}
+
+template <typename SourceT>
+MeshedScreen<SourceT>::~MeshedScreen()
+{
+    for (FieldTrace* t : A) delete t;
+}
+
+template <typename SourceT>
+void MeshedScreen<SourceT>::zero()
+{
+    for (FieldTrace* t : A) t->zero();
+}
+
+template <typename SourceT>
+double MeshedScreen<SourceT>::totalEnergy() const
+{
+    if ((int)area.size() != Np || (int)normal.size() != Np)
+        throw IOexception("MeshedScreen::totalEnergy - cell areas and normals not set.");
+    double total = 0.0;
+    for (int ip = 0; ip < Np; ip++) total -= area[ip] * dot(A[ip]->Poynting(), normal[ip]);
+    return total;
+}
+
+template <typename SourceT>
+void MeshedScreen<SourceT>::integrate()
+{
+    integrate_points(this->source, points, A);
+}
+
+template <typename SourceT>
+double* MeshedScreen<SourceT>::getBuffer()
+{
+    return traces_to_buffer(A, getBufferSize());
+}
+
+template <typename SourceT>
+void MeshedScreen<SourceT>::fromBuffer(double* buffer, std::size_t size)
+{
+    buffer_to_traces(buffer, size, getBufferSize(), A, "MeshedScreen");
+}
+
+// ---- ScreenObserver --------------------------------------------------------
+
+template <typename SourceT>
+ScreenObserver<SourceT>::ScreenObserver(std::string filename, Vector position, Vector dx, Vector dy, unsigned int nx,
+                                        unsigned int ny, double t0, double dt, double dtx, double dty,
+                                        unsigned int nots)
+    : FileName(filename), Origin(position), dX(dx), dY(dy), Nx(nx), Ny(ny), NOTS(nots), t0_obs(t0), dt_obs(dt),
+      dtx_obs(dtx), dty_obs(dty)
+{
+    for (unsigned int ix = 0; ix < Nx; ix++)
+        for (unsigned int iy = 0; iy < Ny; iy++) Traces.push_back(new FieldTrace(CellTimeZero(ix, iy), dt_obs, NOTS));
+}
+
+template <typename SourceT>
+ScreenObserver<SourceT>::~ScreenObserver()
+{
+    for (FieldTrace* t : Traces) delete t;
+}
+
+template <typename SourceT>
+Vector ScreenObserver<SourceT>::CellPosition(unsigned int ix, unsigned int iy) const
+{
+    return Origin + dX * (double)((int)ix - (int)Nx / 2) + dY * (double)((int)iy - (int)Ny / 2);
+}
+
+template <typename SourceT>
+double ScreenObserver<SourceT>::CellTimeZero(unsigned int ix, unsigned int iy) const
+{
+    return t0_obs + dtx_obs * (double)((int)ix - (int)Nx / 2) + dty_obs * (double)((int)iy - (int)Ny / 2);
+}
+
+template <typename SourceT>
+ElMagField ScreenObserver<SourceT>::getField(unsigned int ix, unsigned int iy, unsigned int it)
+{
+    if (ix >= Nx || iy >= Ny || it >= NOTS) throw IOexception("ScreenObserver::getField() - index out of range.");
+    return Traces[ix * Ny + iy]->get_field(it);
+}
+
+template <typename SourceT>
+void ScreenObserver<SourceT>::setField(unsigned int ix, unsigned int iy, unsigned int it, ElMagField field)
+{
+    if (ix >= Nx || iy >= Ny || it >= NOTS) throw IOexception("ScreenObserver::setField() - index out of range.");
+    Traces[ix * Ny + iy]->set_field(it, field);
+}
+
+template <typename SourceT>
+void ScreenObserver<SourceT>::integrate()
+{
+    std::vector<Vector> pts;
+    for (unsigned int ix = 0; ix < Nx; ix++)
+        for (unsigned int iy = 0; iy < Ny; iy++) pts.push_back(CellPosition(ix, iy));
+    integrate_points(this->source, pts, Traces);
+}
+
+template <typename SourceT>
+double* ScreenObserver<SourceT>::getBuffer()
+{
+    return traces_to_buffer(Traces, getBufferSize());
+}
+
+template <typename SourceT>
+void ScreenObserver<SourceT>::fromBuffer(double* buffer, std::size_t size)
+{
+    buffer_to_traces(buffer, size, getBufferSize(), Traces, "ScreenObserver");
+}
+
+template <typename SourceT>
+void ScreenObserver<SourceT>::generateOutput()
+{
+    std::vector<Vector> pts;
+    for (unsigned int ix = 0; ix < Nx; ix++)
+        for (unsigned int iy = 0; iy < Ny; iy++) pts.push_back(CellPosition(ix, iy));
+    writeDump(FileName, pts, Traces);
+}
+
+// ---- PointObserver ---------------------------------------------------------
+
+template <typename SourceT>
+PointObserver<SourceT>::PointObserver(std::string filename, Vector position, double t0, double dt, std::size_t nots)
+    : FileName(filename), Pos(position), trace(new FieldTrace(t0, dt, nots))
+{
+}
+
+template <typename SourceT>
+PointObserver<SourceT>::~PointObserver()
+{
+    delete trace;
+}
+
+template <typename SourceT>
+void PointObserver<SourceT>::integrate()
+{
+    integrate_points(this->source, std::vector<Vector>(1, Pos), std::vector<FieldTrace*>(1, trace));
+}
+
+template <typename SourceT>
+double* PointObserver<SourceT>::getBuffer()
+{
+    return traces_to_buffer(std::vector<FieldTrace*>(1, trace), getBufferSize());
+}
+
+template <typename SourceT>
+void PointObserver<SourceT>::fromBuffer(double* buffer, std::size_t size)
+{
+    buffer_to_traces(buffer, size, getBufferSize(), std::vector<FieldTrace*>(1, trace), "PointObserver");
+}
+
+template class MeshedScreen<Beam>;
+template class MeshedScreen<Bunch>;
+template class ScreenObserver<Beam>;
+template class ScreenObserver<Bunch>;
+template class PointObserver<Beam>;
+template class PointObserver<Bunch>;

@@ -1,0 +1,109 @@
+// screens.h - the three observers of the radiation step (interfaces of
+// MeshedScreen<SourceT> src/mesh.h:99-215, ScreenObserver<SourceT>
+// src/screen.h:100-160 and PointObserver<SourceT> src/point_observer.h:68-110).
+// integrate() hands all pixels of the observer to the GPU in one batched
+// launch instead of the reference's OpenMP pixel loop (src/mesh.cpp:288-342,
+// src/screen.cpp:112-180, src/point_observer.cpp:62-72).  SourceT is Beam or
+// Bunch.  generateOutput() writes a plain binary dump (documented in
+// writeDump); the reference's HDF5/SDDS writers are not part of this library.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "observer.h"
+
+// write "TGDUMP1\0", int32 n_points, int32 n_samples, f64 dt, then per point
+// f64 x,y,z,t0 and the samples [n_samples][6] (little endian)
+void writeDump(const std::string& filename, const std::vector<Vector>& points,
+               const std::vector<FieldTrace*>& traces);
+
+template <typename SourceT>
+class MeshedScreen : public Observer<SourceT> {
+public:
+    // Observation points (cell centres), per-point t0, common dt and Nt; the
+    // cell areas and normals (towards the source) are needed only by
+    // totalEnergy().  The reference reads these from an HDF5 file
+    // (src/mesh.cpp:30-218).
+    MeshedScreen(const std::vector<Vector>& points, const std::vector<double>& t0, double dt, int Nt,
+                 const std::vector<double>& area = std::vector<double>(),
+                 const std::vector<Vector>& normal = std::vector<Vector>(),
+                 const std::string& filename = std::string());
+    ~MeshedScreen() override;
+
+    void init() {}
+    void zero();
+    int get_Np() const { return Np; }
+    int get_Nt() const { return Nt; }
+    double get_dt() const { return dt; }
+    double get_t0(int ip) const { return A[ip]->get_t0(); }
+    Vector get_point(int ip) const { return points[ip]; }
+    FieldTrace* get_trace(int ip) { return A[ip]; }
+    Vector Poynting(int ip) const { return A[ip]->Poynting(); }
+    // -sum_p area_p S_p . normal_p [J] (src/mesh.cpp:744-754)
+    double totalEnergy() const;
+
+    void integrate() override;
+    std::size_t getBufferSize() override { return (std::size_t)Np * Nt * 6; }
+    double* getBuffer() override;
+    void fromBuffer(double* buffer, std::size_t size) override;
+    void generateOutput() override { writeDump(FileName, points, A); }
+
+private:
+    int Np, Nt;
+    double dt;
+    std::vector<Vector> points;
+    std::vector<double> area;
+    std::vector<Vector> normal;
+    std::vector<FieldTrace*> A;
+    std::string FileName;
+};
+
+template <typename SourceT>
+class ScreenObserver : public Observer<SourceT> {
+public:
+    // nx x ny cells at position + dx (ix - nx/2) + dy (iy - ny/2), trace start
+    // t0 + dtx (ix - nx/2) + dty (iy - ny/2) (integer division, src/screen.cpp:98-110)
+    ScreenObserver(std::string filename, Vector position, Vector dx, Vector dy, unsigned int nx, unsigned int ny,
+                   double t0, double dt, double dtx, double dty, unsigned int nots);
+    ~ScreenObserver() override;
+
+    Vector CellPosition(unsigned int ix, unsigned int iy) const;
+    double CellTimeZero(unsigned int ix, unsigned int iy) const;
+    ElMagField getField(unsigned int ix, unsigned int iy, unsigned int it);
+    void setField(unsigned int ix, unsigned int iy, unsigned int it, ElMagField field);
+
+    void integrate() override;
+    std::size_t getBufferSize() override { return (std::size_t)Nx * Ny * NOTS * 6; }
+    double* getBuffer() override;
+    void fromBuffer(double* buffer, std::size_t size) override;
+    void generateOutput() override;
+
+private:
+    std::string FileName;
+    Vector Origin, dX, dY;
+    unsigned int Nx, Ny, NOTS;
+    double t0_obs, dt_obs, dtx_obs, dty_obs;
+    std::vector<FieldTrace*> Traces;   // [ix * Ny + iy]
+};
+
+template <typename SourceT>
+class PointObserver : public Observer<SourceT> {
+public:
+    PointObserver(std::string filename, Vector position, double t0, double dt, std::size_t nots);
+    ~PointObserver() override;
+
+    void integrate() override;
+    ElMagField getField(std::size_t idx) { return trace->get_field(idx); }
+    void setField(std::size_t idx, ElMagField f) { trace->set_field(idx, f); }
+    FieldTrace* getTrace() { return trace; }
+    std::size_t getBufferSize() override { return 6 * trace->get_N(); }
+    double* getBuffer() override;
+    void fromBuffer(double* buffer, std::size_t size) override;
+    void generateOutput() override { writeDump(FileName, std::vector<Vector>(1, Pos), std::vector<FieldTrace*>(1, trace)); }
+
+private:
+    std::string FileName;
+    Vector Pos;
+    FieldTrace* trace;
+};
