@@ -1,0 +1,285 @@
+// host_api_test.cpp - the reference's own radiation tests, written against the
+// C++ host API (teufel_amd/host) and run on the GPU.
+//
+//   host_api_test kat [fp64|mixed]
+//       the five deposit cases of tests/teufel.integrate_field.cpp:91-191
+//       (pseudo-particles with injected retarded times), same expected values
+//       and the same 1e-6 tolerance
+//   host_api_test loop [fp64|mixed]
+//       tests/teufel.loop.cpp:160-190: the field at the centre of a circular
+//       orbit seen by a PointObserver<Bunch> equals mu0 I / (2R) within 1e-3.
+//       The orbit (one revolution, 200 steps) is the exact circle instead of
+//       the reference's Vay-tracked one.
+//   host_api_test screens <input> <mesh.dump> <screen.dump> [fp64|mixed]
+//       trajectories and a mesh definition from <input> (written by
+//       tests/test_host_api.py), integrated by MeshedScreen<Beam> (two
+//       bunches) and ScreenObserver<Bunch>; both dumps are compared against
+//       the oracle by the Python test.  Prints the mesh's totalEnergy().
+//
+// Exit status: number of failed checks (0 = pass), 100+ on usage/IO errors.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bunch.h"
+#include "global.h"
+#include "gpu_radiation.h"
+#include "screens.h"
+
+namespace {
+
+int set_precision(const char* arg)
+{
+    if (!arg || std::strcmp(arg, "fp64") == 0) {
+        teufel_gpu::set_precision(0);
+        return 0;
+    }
+    if (std::strcmp(arg, "mixed") == 0) {
+        teufel_gpu::set_precision(1);
+        return 0;
+    }
+    std::fprintf(stderr, "unknown precision %s\n", arg);
+    return 100;
+}
+
+// ---- deposit known answers ---------------------------------------------------
+
+struct Pseudo {
+    double t1, t2, e1, e2;
+};
+
+int kat_case(const char* name, const std::vector<Pseudo>& ps, double t0, double dt, int n,
+             const std::vector<double>& expected)
+{
+    std::vector<double> ts, f;
+    for (const Pseudo& p : ps) {
+        ts.push_back(p.t1);
+        ts.push_back(p.t2);
+        double f1[6] = {p.e1, 0, 0, 0, 0, 0}, f2[6] = {p.e2, 0, 0, 0, 0, 0};
+        f.insert(f.end(), f1, f1 + 6);
+        f.insert(f.end(), f2, f2 + 6);
+    }
+    FieldTrace ft(t0, dt, (std::size_t)n);
+    teufel_gpu::deposit_only((int)ps.size(), 2, ts.data(), f.data(), &ft);
+    int errors = 0;
+    std::printf("%s:\n", name);
+    for (int i = 0; i < n; i++) {
+        double ex = ft.get_field((std::size_t)i).E().x;
+        bool bad = std::fabs(ex - expected[i]) > 1e-6;
+        errors += bad;
+        std::printf("  t=%6.2f : Ex=%9.6f%s\n", ft.get_time((std::size_t)i), ex, bad ? "  <-- expected" : "");
+    }
+    return errors;
+}
+
+int run_kat()
+{
+    int errors = 0;
+    errors += kat_case("aligned", {{0.0, 1.0, 1.5, 0.5}}, 0.0, 0.2, 8, {0.725, 1.3, 1.1, 0.9, 0.7, 0.275, 0.0, 0.0});
+    errors += kat_case("shifted", {{0.0, 1.0, 1.5, 0.5}}, -0.1, 0.2, 8, {0.0, 1.4, 1.2, 1.0, 0.8, 0.6, 0.0, 0.0});
+    errors += kat_case("spike a", {{0.05, 0.15, 1.5, 0.5}}, -0.2, 0.2, 4, {0.0, 0.3125, 0.1875, 0.0});
+    errors += kat_case("spike b", {{0.05, 0.15, 1.5, 0.5}}, -0.1, 0.2, 4, {0.0, 0.5, 0.0, 0.0});
+    errors += kat_case("five overlapping",
+                       {{1.0, 2.0, 1.0, 0.0}, {1.0, 2.0, 0.0, 1.0}, {2.0, 2.2, 1.0, 1.0}, {2.2, 2.4, 1.0, 0.0},
+                        {2.4, 3.0, 0.0, 1.0}},
+                       0.0, 0.5, 8, {0.0, 0.0, 0.5, 1.0, 0.9875, 0.316667, 0.395833, 0.0});
+    return errors;
+}
+
+// ---- circular loop -------------------------------------------------------------
+
+int run_loop()
+{
+    const double B0 = 0.5, gamma = 10.0;
+    const double bg = std::sqrt(gamma * gamma - 1.0);
+    const double omega = B0 / (gamma * mecsquared) * SpeedOfLight * SpeedOfLight;
+    const double radius = bg * mecsquared / (B0 * SpeedOfLight);
+    const double tau = 2 * Pi / omega;
+    const double charge = 1.0e-9;
+    const double ne = charge / ElementaryCharge;
+    const double expected = MuNull * charge / tau / (2.0 * radius);
+    // positive charge moving along +z in B = +y: circle about (-R, 0, 0) in x-z
+    ChargedParticle* e = new ChargedParticle(ne, ne);
+    const int steps = 200;
+    for (int k = 0; k <= steps; k++) {
+        double t = tau * k / steps, ph = omega * t;
+        Vector X(-radius + radius * std::cos(ph), 0.0, radius * std::sin(ph));
+        Vector P(-bg * std::sin(ph), 0.0, bg * std::cos(ph));
+        Vector A(-bg * omega * std::cos(ph), 0.0, -bg * omega * std::sin(ph));
+        if (k == 0)
+            e->initTrajectory(t, X, P, A);
+        else
+            e->storeTrajectoryPoint(t, X, P, A);
+    }
+    Bunch* bunch = new Bunch();
+    bunch->Add(e);
+    PointObserver<Bunch> obs("", Vector(-radius, 0.0, 0.0), tau - 1.0e-15, 2.0e-15, 2);
+    obs.setSource(bunch);
+    obs.integrate();
+    double b = obs.getField(0).B().norm();
+    int errors = std::fabs(b - expected) / expected > 1e-3;
+    std::printf("loop: R = %.6g m, tau = %.6g s, |B| = %.6g T, expected %.6g T%s\n", radius, tau, b, expected,
+                errors ? "  <-- failed" : "");
+    // buffer exchange round trip (src/observer.h getBuffer / fromBuffer)
+    std::size_t n = obs.getBufferSize();
+    std::unique_ptr<double[]> buf(obs.getBuffer());
+    PointObserver<Bunch> copy("", Vector(-radius, 0.0, 0.0), tau - 1.0e-15, 2.0e-15, 2);
+    copy.fromBuffer(buf.get(), n);
+    if (copy.getField(0).B().norm() != b) {
+        std::printf("loop: getBuffer/fromBuffer round trip differs\n");
+        errors++;
+    }
+    delete bunch;
+    return errors;
+}
+
+// ---- screens from an input file -----------------------------------------------
+
+struct Reader {
+    FILE* f;
+    bool ok = true;
+    template <class T>
+    void get(T* p, size_t n)
+    {
+        ok = ok && std::fread(p, sizeof(T), n, f) == n;
+    }
+    template <class T>
+    T one()
+    {
+        T v{};
+        get(&v, 1);
+        return v;
+    }
+    Vector vec()
+    {
+        double v[3] = {0, 0, 0};
+        get(v, 3);
+        return Vector(v[0], v[1], v[2]);
+    }
+};
+
+int run_screens(const char* input, const char* mesh_out, const char* screen_out)
+{
+    FILE* f = std::fopen(input, "rb");
+    if (!f) {
+        std::fprintf(stderr, "cannot open %s\n", input);
+        return 101;
+    }
+    Reader r{f};
+    int32_t np = r.one<int32_t>(), npts = r.one<int32_t>();
+    if (!r.ok || np < 1 || npts < 1) return 102;
+    size_t n = (size_t)np * npts;
+    std::vector<double> a(10 * n), q(np);
+    r.get(a.data(), a.size());
+    r.get(q.data(), q.size());
+    int32_t npix = r.one<int32_t>(), nt = r.one<int32_t>();
+    double dt = r.one<double>();
+    if (!r.ok || npix < 1 || nt < 1) return 103;
+    std::vector<Vector> pts, nrm;
+    std::vector<double> t0(npix), area(npix);
+    for (int i = 0; i < npix; i++) pts.push_back(r.vec());
+    r.get(t0.data(), npix);
+    r.get(area.data(), npix);
+    for (int i = 0; i < npix; i++) nrm.push_back(r.vec());
+    Vector origin = r.vec(), dx = r.vec(), dy = r.vec();
+    int32_t nx = r.one<int32_t>(), ny = r.one<int32_t>();
+    double st0 = r.one<double>(), sdt = r.one<double>(), dtx = r.one<double>(), dty = r.one<double>();
+    int32_t nots = r.one<int32_t>();
+    std::fclose(f);
+    if (!r.ok) return 104;
+
+    // particles: charge in e, trajectory rows [quantity][particle][point]
+    const double* t = a.data();
+    auto make = [&](int i) {
+        ChargedParticle* p = new ChargedParticle(q[i], 1.0);
+        for (int k = 0; k < npts; k++) {
+            size_t g = (size_t)i * npts + k;
+            Vector X(t[n + g], t[2 * n + g], t[3 * n + g]);
+            Vector P(t[4 * n + g], t[5 * n + g], t[6 * n + g]);
+            Vector A(t[7 * n + g], t[8 * n + g], t[9 * n + g]);
+            if (k == 0)
+                p->initTrajectory(t[g], X, P, A);
+            else
+                p->storeTrajectoryPoint(t[g], X, P, A);
+        }
+        return p;
+    };
+    // a Beam of two bunches (first half, second half) and one Bunch of all
+    Beam* beam = new Beam();
+    Bunch* b1 = new Bunch();
+    Bunch* b2 = new Bunch();
+    Bunch* all = new Bunch();
+    for (int i = 0; i < np; i++) {
+        (i < (np + 1) / 2 ? b1 : b2)->Add(make(i));
+        all->Add(make(i));
+    }
+    beam->Add(b1);
+    beam->Add(b2);
+
+    MeshedScreen<Beam> mesh(pts, t0, dt, nt, area, nrm, mesh_out);
+    mesh.setSource(beam);
+    mesh.integrate();
+    mesh.generateOutput();
+    std::printf("mesh: %d points x %d samples, totalEnergy = %.17g J\n", npix, nt, mesh.totalEnergy());
+
+    ScreenObserver<Bunch> screen(screen_out, origin, dx, dy, (unsigned)nx, (unsigned)ny, st0, sdt, dtx, dty,
+                                 (unsigned)nots);
+    screen.setSource(all);
+    screen.integrate();
+    screen.generateOutput();
+    std::printf("screen: %d x %d cells x %d samples\n", nx, ny, nots);
+
+    int errors = 0;
+    // integrate() accumulates: a second pass doubles the traces
+    std::unique_ptr<double[]> once(screen.getBuffer());
+    screen.integrate();
+    std::unique_ptr<double[]> twice(screen.getBuffer());
+    for (size_t i = 0; i < screen.getBufferSize(); i++)
+        if (std::fabs(twice[i] - 2 * once[i]) > 1e-12 * std::fabs(2 * once[i]) + 1e-300) {
+            errors++;
+            std::printf("screen: second integrate() does not accumulate at %zu\n", i);
+            break;
+        }
+    screen.fromBuffer(once.get(), screen.getBufferSize());
+    if (screen.getField(0, 0, 0).E().x != once[0]) errors++;
+    delete beam;
+    delete all;
+    return errors;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s kat|loop|screens ...\n", argv[0]);
+        return 100;
+    }
+    const std::string mode = argv[1];
+    try {
+        int errors;
+        if (mode == "kat") {
+            if (set_precision(argc > 2 ? argv[2] : nullptr)) return 100;
+            errors = run_kat();
+        } else if (mode == "loop") {
+            if (set_precision(argc > 2 ? argv[2] : nullptr)) return 100;
+            errors = run_loop();
+        } else if (mode == "screens" && argc >= 5) {
+            if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
+            errors = run_screens(argv[2], argv[3], argv[4]);
+        } else {
+            std::fprintf(stderr, "bad arguments\n");
+            return 100;
+        }
+        teufel_gpu::shutdown();
+        std::printf("%s: errors %d - %s\n", mode.c_str(), errors, errors ? "test failed" : "OK");
+        return errors;
+    } catch (const std::exception& e) {
+        std::printf("%s: exception: %s\n", mode.c_str(), e.what());
+        return 110;
+    }
+}

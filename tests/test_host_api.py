@@ -1,0 +1,151 @@
+"""The C++ host API (teufel_amd/host: ChargedParticle, Bunch, Beam, FieldTrace,
+MeshedScreen, ScreenObserver, PointObserver) driven by tests/cpp/host_api_test.cpp.
+
+CPU: the host library and the test program build and link.
+GPU: the program runs the reference's own deposit known answers
+(tests/teufel.integrate_field.cpp) and loop check (tests/teufel.loop.cpp)
+through the host classes, and integrates a MeshedScreen<Beam> and a
+ScreenObserver<Bunch> whose dumps are compared with the oracle and the
+reference fixture (FP64: eps_E <= 1e-12; mixed: eps_W <= 1e-5).
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from teufel_amd import host_build
+from teufel_amd import workloads as W
+from teufel_amd.metrics import energy_rel_error, trace_rel_error
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp")
+BIN = os.path.join(ROOT, "tests", "cpp", "_build", "host_api_test")
+
+
+def build_test_program():
+    """Host library and test program (g++ only; the HIP library must exist)."""
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    host_build.build()
+    return host_build.build_program(SRC, BIN)
+
+
+def _program():
+    if not os.path.exists(BIN) or os.path.getmtime(BIN) < os.path.getmtime(SRC):
+        build_test_program()
+    return BIN
+
+
+def run(*args, timeout=120):
+    print("\n[case] host_api_test " + " ".join(args), flush=True)
+    r = subprocess.run([_program()] + list(args), capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+def test_host_library_builds_and_links():
+    from teufel_amd import build as B
+    if not os.path.exists(os.path.join(ROOT, "teufel_amd", "lib", "libteufel_gpu.so")):
+        B.build()
+    assert os.path.exists(build_test_program())
+
+
+def pixel_energy(traces, dt, normal):
+    """W_p = -S_p . n_p with S_p = dt sum_i E_i x B_i / mu0."""
+    S = np.array([oracle.trace_poynting(t, dt) for t in traces])
+    return W.energy_density(S, np.broadcast_to(normal, S.shape))
+
+
+def read_dump(path):
+    with open(path, "rb") as f:
+        raw = f.read()
+    assert raw[:8] == b"TGDUMP1\0"
+    n_pt, n_s = struct.unpack_from("<ii", raw, 8)
+    (dt,) = struct.unpack_from("<d", raw, 16)
+    off = 24
+    head = np.empty((n_pt, 4))
+    traces = np.empty((n_pt, n_s, 6))
+    for i in range(n_pt):
+        head[i] = np.frombuffer(raw, "<f8", 4, off)
+        off += 32
+        traces[i] = np.frombuffer(raw, "<f8", 6 * n_s, off).reshape(n_s, 6)
+        off += 48 * n_s
+    assert off == len(raw)
+    return head[:, :3], head[:, 3], dt, traces
+
+
+def write_input(path, tr, obs, t0, dt, nt, area, normal, scr):
+    np_, npts = tr.t.shape
+    with open(path, "wb") as f:
+        f.write(struct.pack("<ii", np_, npts))
+        for name in oracle.Trajectories.NAMES:
+            f.write(np.ascontiguousarray(getattr(tr, name), "<f8").tobytes())
+        f.write(np.ascontiguousarray(tr.charge, "<f8").tobytes())
+        f.write(struct.pack("<iid", obs.shape[0], nt, dt))
+        for a in (obs, t0, area, normal):
+            f.write(np.ascontiguousarray(a, "<f8").tobytes())
+        f.write(np.asarray(scr["origin"] + scr["dx"] + scr["dy"], "<f8").tobytes())
+        f.write(struct.pack("<ii", scr["nx"], scr["ny"]))
+        f.write(struct.pack("<dddd", scr["t0"], scr["dt"], scr["dtx"], scr["dty"]))
+        f.write(struct.pack("<i", scr["nots"]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_kats(precision):
+    r = run("kat", precision)
+    assert r.returncode == 0
+
+
+@pytest.mark.gpu
+def test_host_loop_point_observer():
+    r = run("loop", "fp64")
+    assert r.returncode == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_screens_vs_oracle(tmp_path, precision):
+    g = np.load(os.path.join(ROOT, "tests", "golden", "bunch_ref.npz"))
+    tr = oracle.Trajectories(**{n: g[n] for n in oracle.Trajectories.NAMES}, charge=g["charge"])
+    # MeshedScreen: the fixture's 4 x 4 pixels (0.05 m pitch at 10 m, per-pixel t0)
+    obs, t0, dt, nt = g["obs"], g["t0"], float(g["dt"]), int(g["n_bins"])
+    area = np.full(obs.shape[0], 0.05 * 0.05)
+    normal = np.tile([0.0, 0.0, -1.0], (obs.shape[0], 1))
+    # ScreenObserver: 9 x 7 cells around the axis at 10 m, linear time offsets
+    axis = int(np.flatnonzero((obs[:, 0] == 0) & (obs[:, 1] == 0))[0])
+    scr = dict(origin=[0.0, 0.0, 10.0], dx=[0.004, 0.0, 0.0], dy=[0.0, 0.005, 0.0], nx=9, ny=7,
+               t0=float(t0[axis]), dt=dt, dtx=1e-15, dty=-2e-15, nots=nt)
+    inp = str(tmp_path / "in.bin")
+    mesh_dump, scr_dump = str(tmp_path / "mesh.dump"), str(tmp_path / "screen.dump")
+    write_input(inp, tr, obs, t0, dt, nt, area, normal, scr)
+    r = run("screens", inp, mesh_dump, scr_dump, precision)
+    assert r.returncode == 0
+
+    pts, pt0, pdt, mesh = read_dump(mesh_dump)
+    assert np.array_equal(pts, obs) and np.array_equal(pt0, t0) and pdt == dt
+    ref = oracle.integrate_screen(tr, obs, t0, dt, nt)
+    if precision == "fp64":
+        assert trace_rel_error(mesh, ref) <= 1e-12
+        assert trace_rel_error(mesh, g["traces"]) <= 1e-12
+    else:
+        assert energy_rel_error(pixel_energy(mesh, dt, normal), pixel_energy(ref, dt, normal))[0] <= 1e-5
+    e_line = [ln for ln in r.stdout.splitlines() if "totalEnergy" in ln][0]
+    energy = float(e_line.split("=")[1].split()[0])
+    e_ref = oracle.total_energy(ref, dt, area, normal)
+    assert abs(energy - e_ref) <= (1e-12 if precision == "fp64" else 1e-5) * abs(e_ref)
+
+    spts, st0, sdt, straces = read_dump(scr_dump)
+    assert straces.shape == (63, scr["nots"], 6) and sdt == scr["dt"]
+    ix, iy = np.meshgrid(np.arange(9) - 9 // 2, np.arange(7) - 7 // 2, indexing="ij")
+    assert np.allclose(spts[:, 0], 0.004 * ix.ravel(), rtol=0, atol=1e-15)
+    assert np.allclose(spts[:, 1], 0.005 * iy.ravel(), rtol=0, atol=1e-15)
+    assert np.allclose(st0, scr["t0"] + 1e-15 * ix.ravel() - 2e-15 * iy.ravel(), rtol=0, atol=1e-22)
+    sref = oracle.integrate_screen(tr, spts, st0, scr["dt"], scr["nots"])
+    if precision == "fp64":
+        assert trace_rel_error(straces, sref) <= 1e-12
+    else:
+        n = np.array([0.0, 0.0, -1.0])
+        assert energy_rel_error(pixel_energy(straces, sdt, n), pixel_energy(sref, sdt, n))[0] <= 1e-5
