@@ -6,14 +6,27 @@ tg_integrate of the workload (all particles x all pixels x all bins) with the
 trajectories already resident in HBM (generated on the device by the Vay
 pusher before timing).
 
-Default workload: c3 (mixed precision): 1e5 electrons, 4000 steps, 128 x 128
-pixel flat screen at 10 m, 2048 bins.  The FP64 kernel is timed on the same
-workload and the mixed result's per-pixel energy error against it is reported.
+Default workload: c2, the configuration the metric is quoted on (1e4
+electrons, 4000 steps, 64 x 64 pixel flat screen at 10 m, 1024 bins).  The
+mixed-precision kernel gives `value`; the FP64 kernel is timed on the same
+workload (fp64_value, mixed_over_fp64) and the mixed result's per-pixel
+energy error against it is reported (eps_W_mixed_vs_fp64, bound 1e-5).  c3
+(1e5 electrons, 128 x 128 pixels, 2048 bins: ~2.5 min per mixed step) and c4
+(32 uneven screens in one launch) run with --workload.
 The CPU baseline is the reference's own compiled integrateFieldTrace
-(oracle/_ref) driven by an OpenMP pixel loop on the host cores, timed on a
-bounded sample and scaled by pairs.
+(oracle/_ref) driven by an OpenMP pixel loop on the host cores (the oracle
+restatement where the reference build is absent), timed on a bounded sample
+of the same workload and scaled by pairs.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+roofline: the radiation kernel's algorithmic FP operations (106 per pair) per
+launch / its mean launch duration from HIP events on the launch stream,
+against the vector peak of the precision mix (VALU-bound: no MFMA shape).
+traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, read
+from the summary file profiles/traffic_<kernel>_<workload>.txt when present.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
+       python bench.py --calibrate-copy BYTES   (a plain copy of BYTES, for
+                                                 scaling the PMC byte counters)
 """
 import argparse
 import json
@@ -36,6 +49,8 @@ FLOP_PER_PAIR = 106.0          # algorithmic FP ops per (point, pixel): 72 LW + 
 PEAK_FP64 = 78.6e12            # MI355X FP64 vector
 PEAK_FP32 = 157.3e12           # MI355X FP32 vector (published, packed)
 FP64_OPS_MIXED = 23.0          # of the 106, done in FP64 by the mixed kernel
+METRIC = "LW field evals/s (particle-step\u00d7pixel), mixed vs FP64 GPU at \u22641e-5 rel energy"
+KERNEL_NAME = {TG_MIXED: "radiation_kernel<1>", TG_FP64: "radiation_kernel<0>"}
 
 
 def mixed_peak():
@@ -114,21 +129,51 @@ def cpu_baseline(ctx, w, target_s=15.0, max_particles=64):
                          s.n_bins, dt1)}
 
 
+def calibrate_copy(nbytes):
+    """One elementwise copy kernel reading and writing nbytes each (the known
+    size the FETCH_SIZE / WRITE_SIZE counters are scaled by)."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    log("copy kernel: read %d B, wrote %d B" % (4 * n, 4 * n))
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), from the committed summary."""
+    tag = kernel.replace("<", "").replace(">", "")
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (tag, workload))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--workload", default="c2")
     ap.add_argument("--particles", type=int, default=None, help="shrink the workload (testing only)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the launch sequence from a hipGraph")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the radiation step shards by pixel tiles "
               "across GPUs by design, but no multi-GPU path is built); use --gpus 1", file=sys.stderr)
@@ -170,6 +215,10 @@ def main():
         if not args.no_autotune:
             ctx.autotune(TG_FP64)
         w64, k64, t64 = time_integrate(ctx, TG_FP64, args.fp64_steps, 0)
+        # one clean pass of each precision for the error (timed passes accumulate)
+        for i in ids:
+            ctx.zero_screen(i)
+        ctx.integrate(TG_FP64)
         S64 = [ctx.screen_poynting(i) for i in ids]
         T64 = [ctx.read_traces(i) for i in ids]
         for i in ids:
@@ -194,7 +243,7 @@ def main():
         extra["gpu_over_cpu"] = value / cpu["value"]
     s0 = w.screens[0]
     line = {
-        "metric": "LW field evals/s (particle-step x pixel), %s precision" % args.precision,
+        "metric": METRIC,
         "value": value,
         "unit": "LW field evals/s",
         "n_gpus": 1,
@@ -211,8 +260,8 @@ def main():
                    "bins": s0.n_bins, "screens": len(w.screens), "n_pairs": n_pairs,
                    "precision": args.precision, "tuning": tuned, "graph": bool(args.graph)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "flop_per_pair": FLOP_PER_PAIR, "kernel_ms": kern * 1e3},
+                     "frac": achieved / peak, "traffic": read_traffic(KERNEL_NAME[prec], w.name),
+                     "kernel": KERNEL_NAME[prec], "flop_per_pair": FLOP_PER_PAIR, "kernel_ms": kern * 1e3},
         "cpu_baseline": cpu,
     }
     line.update(extra)
