@@ -57,6 +57,8 @@ typedef struct {
     int particles_per_chunk; /* particles per work item (partial traces summed in order) */
     int use_graph;           /* 1: replay the launch sequence from a captured hipGraph */
     int window_groups;       /* windows of a (tile, chunk) split into this many work items */
+    int walk;                /* 1: the wave walks each particle's steps together (lanes share a
+                                step); 2: every lane (pixel) walks its own steps; 0: default (2) */
 } tg_tuning;
 
 /* Context on one device; one trace store and one trajectory set per context.

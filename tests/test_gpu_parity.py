@@ -97,23 +97,25 @@ def _small_bunch(n_particles=48, n_steps=4000, npx=16, n_bins=256, seed=77):
     return tr, obs, t0, dt, nrm
 
 
-@pytest.mark.parametrize("window,chunk,groups", [(8, 0, 1), (4, 7, 3), (16, 1000, 2), (32, 16, 5), (6, 0, 0)])
-def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk, groups):
-    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d groups=%d" % (window, chunk, groups))
+@pytest.mark.parametrize("window,chunk,groups,walk", [(8, 0, 1, 2), (4, 7, 3, 2), (16, 1000, 2, 1), (32, 16, 5, 2),
+                                                      (6, 0, 0, 1), (12, 0, 0, 2), (5, 3, 4, 1)])
+def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk, groups, walk):
+    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d groups=%d walk=%d" % (window, chunk, groups, walk))
     tr, obs, t0, dt, _ = _small_bunch()
     ref = oracle.integrate_screen(tr, obs, t0, dt, 256)
     ctx.remove_screens()
     ctx.set_trajectories(tr)
     sid = ctx.add_screen(obs, t0, dt, 256)
     ctx.integrate(TG_FP64, tuning={"bins_per_window": window, "particles_per_chunk": chunk,
-                                   "window_groups": groups})
+                                   "window_groups": groups, "walk": walk})
     err = trace_rel_error(ctx.read_traces(sid), ref)
     print("eps_E = %.3e" % err)
     assert err <= FP64_TOL
 
 
-def test_mixed_energy_error_and_determinism(ctx):
-    say("48 e- mixed vs FP64, energy per pixel; repeat bitwise")
+@pytest.mark.parametrize("walk", [1, 2])
+def test_mixed_energy_error_and_determinism(ctx, walk):
+    say("48 e- mixed vs FP64, energy per pixel; repeat bitwise; walk=%d" % walk)
     tr, obs, t0, dt, nrm = _small_bunch()
     ctx.remove_screens()
     ctx.set_trajectories(tr)
@@ -122,17 +124,39 @@ def test_mixed_energy_error_and_determinism(ctx):
     S64 = ctx.screen_poynting(sid)
     T64 = ctx.read_traces(sid)
     ctx.zero_screen(sid)
-    ctx.integrate(TG_MIXED)
+    ctx.integrate(TG_MIXED, tuning={"walk": walk})
     Tm = ctx.read_traces(sid)
     Sm = ctx.screen_poynting(sid)
     ctx.zero_screen(sid)
-    ctx.integrate(TG_MIXED)
+    ctx.integrate(TG_MIXED, tuning={"walk": walk})
     Tm2 = ctx.read_traces(sid)
     assert np.array_equal(Tm, Tm2)
     epsW, excl = energy_rel_error(W.energy_density(Sm, nrm), W.energy_density(S64, nrm))
     epsE = trace_rel_error(Tm, T64)
     print("mixed: eps_W = %.3e (excluded %.3f), eps_E = %.3e" % (epsW, excl, epsE))
     assert epsW <= MIXED_ENERGY_TOL
+
+
+@pytest.mark.parametrize("prec", [TG_FP64, TG_MIXED])
+def test_padded_trajectories_add_zeros(ctx, prec):
+    say("particles padded by repeating their last point (zero-length steps), prec=%d" % prec)
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=6, n_steps=4000, npx=8, n_bins=128)
+    cut = [4001, 2800, 4001, 1400, 2, 3601]
+    for p, n in enumerate(cut):
+        for name in oracle.Trajectories.NAMES:
+            a = getattr(tr, name)
+            a[p, n:] = a[p, n - 1]
+    ref = oracle.integrate_screen(tr, obs, t0, dt, 128)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 128)
+    ctx.integrate(prec)
+    out = ctx.read_traces(sid)
+    assert np.all(np.isfinite(out))
+    if prec == TG_FP64:
+        assert trace_rel_error(out, ref) <= FP64_TOL
+    else:
+        assert trace_rel_error(out, ref) <= 1e-4
 
 
 def test_gpu_pusher_matches_oracle(ctx):

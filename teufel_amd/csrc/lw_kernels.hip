@@ -163,21 +163,31 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
     o.f[5] = (nx * ey - ny * ex) * kInvSpeedOfLight;
 }
 
+// Retarded time and bucket index of the mixed kernel, also returning 1/R:
+// 1/sqrt from the FP32 hardware estimate and one FP64 Newton step (~2^-46
+// relative).  Every FMA is explicit, so every caller gets the same doubles.
+__device__ __forceinline__ void timing_mixed(double t, double dx, double dy, double dz, double t0,
+                                             double inv_dt, double& y, double& ts, int& idx)
+{
+#pragma clang fp contract(off)
+    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+    y = (double)__builtin_amdgcn_rsqf((float)r2);
+    const double h = r2 * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+    const double R = r2 * y;
+    ts = __builtin_fma(R, kInvSpeedOfLight, t);
+    idx = round_index((ts - t0) * inv_dt);
+}
+
 // Mixed precision: FP64 distance, retarded time and the n_z beta_z part of
 // 1 - n.beta (the cancelling term), FP32 for everything else.
 __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double Py, double Pz,
                                            double t0, double dt, double inv_dt, PointT<float>& o)
 {
     const double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    // 1/sqrt: FP32 hardware estimate + one FP64 Newton step (~2^-46 relative)
-    double y = (double)__builtin_amdgcn_rsqf((float)r2);
-    const double h = r2 * y;
-    const double e = __builtin_fma(-h, y, 1.0);
-    y = __builtin_fma(0.5 * y, e, y);
-    const double R = r2 * y;
-    o.ts = __builtin_fma(R, kInvSpeedOfLight, r.t);
-    o.idx = round_index((o.ts - t0) * inv_dt);
+    double y;
+    timing_mixed(r.t, dx, dy, dz, t0, inv_dt, y, o.ts, o.idx);
     const double nz = dz * y;
     const float fy = (float)y;
     const float nx = (float)dx * fy, ny = (float)dy * fy, fnz = (float)nz;
@@ -468,6 +478,92 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
     return steps;
 }
 
+// Bucket index of trajectory point k seen from P, computed exactly as the
+// full evaluation of that precision computes it (t, x, y, z lead both record
+// types).
+template <int PREC>
+__device__ __forceinline__ int point_index(const typename RecOf<PREC>::type* __restrict__ rp, int k, double Px,
+                                           double Py, double Pz, double t0, double dt, double inv_dt)
+{
+    const typename RecOf<PREC>::type& r = rp[k];
+    int idx;
+    if constexpr (PREC == 0) {
+        double R, ts;
+        timing_exact(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, dt, inv_dt, R, ts, idx);
+    } else {
+        double y, ts;
+        timing_mixed(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, inv_dt, y, ts, idx);
+    }
+    return idx;
+}
+
+// Walk of one particle for the window [b0, b1) in which every lane (pixel)
+// follows its own step: from the wave-uniform start k each lane first skips,
+// with the retarded time alone, the steps that end before b0 (they deposit
+// nothing in the window), then evaluates and deposits its steps until one
+// ends at or beyond b1 (that crossing step is deposited here and again in the
+// next window).  Records are read per lane, one step ahead.  Lanes thus spend
+// full evaluations only on their own steps; the wave runs as long as its
+// longest lane.  Returns the main-loop trip count; nk = the smallest stopping
+// step over the lanes (the next window's start), nmin = the smallest bucket
+// at those steps.  With verify, k is a searched guess: step back until no
+// lane starts at or beyond b0 at point k.
+template <int PREC>
+__device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __restrict__ rp, int k, int NP,
+                                          bool active, double Px, double Py, double Pz, double t0, double dt,
+                                          double inv_dt, double tmin, double tmax, int n_bins, int b0, int b1,
+                                          int W, const Window<typename RecOf<PREC>::amp>& win, bool verify,
+                                          int& nk, int& nmin)
+{
+    using F = typename RecOf<PREC>::amp;
+    using Rec = typename RecOf<PREC>::type;
+    if (verify) {
+        int back = 32;
+        while (k > 0 && __ballot(active && point_index<PREC>(rp, k, Px, Py, Pz, t0, dt, inv_dt) >= b0)) {
+            k = k > back ? k - back : 0;
+            back *= 2;
+        }
+    }
+    int kl = k;
+    bool skip = active && kl < NP - 1;
+    while (__ballot(skip)) {
+        if (skip) {
+            if (point_index<PREC>(rp, kl + 1, Px, Py, Pz, t0, dt, inv_dt) < b0) {
+                ++kl;
+                skip = kl < NP - 1;
+            } else {
+                skip = false;
+            }
+        }
+    }
+    PointT<F> p0, p1;
+    eval_point<PREC>(rp[kl], Px, Py, Pz, t0, dt, inv_dt, p0);
+    bool walking = active && kl < NP - 1 && p0.idx < b1;
+    Rec r1 = rp[kl + 1 < NP ? kl + 1 : NP - 1];
+    int steps = 0;
+    while (__ballot(walking)) {
+        const Rec rn = rp[kl + 2 < NP ? kl + 2 : NP - 1];
+        eval_point<PREC>(r1, Px, Py, Pz, t0, dt, inv_dt, p1);
+        deposit_window<F>(win, W, walking, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        ++steps;
+        if (walking) {
+            if (p1.idx >= b1) {
+                walking = false;
+            } else {
+                ++kl;
+                p0 = p1;
+                walking = kl < NP - 1;
+            }
+        }
+        r1 = rn;
+    }
+    const bool stopped = active && kl < NP - 1;
+    const int k_next = wave_min(stopped ? kl : INT_MAX);
+    nk = k_next == INT_MAX ? NP - 1 : k_next;
+    nmin = wave_min(stopped ? p0.idx : INT_MAX);
+    return steps;
+}
+
 // First trajectory point of each particle of a batch whose bucket for one
 // representative pixel is >= b: lanes = particles, binary search over the
 // monotone retarded time.  The result seeds a cursor that the walk verifies.
@@ -491,7 +587,7 @@ __device__ __forceinline__ int search_first_point(const typename RecOf<PREC>::ty
     return lo;
 }
 
-template <int PREC>
+template <int PREC, bool LANES>
 __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items,
                                                         const ScreenDev* __restrict__ screens,
                                                         const typename RecOf<PREC>::type* __restrict__ recs,
@@ -556,8 +652,13 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                 const bool verify = __builtin_amdgcn_readlane(cur.y, j) == INT_MIN && k > 0;
                 const int p = pb + j;
                 int nk, nmin;
-                n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
-                                               tmin, tmax, nb, b0, b1, W, win, stage, lane, verify, nk, nmin);
+                if constexpr (LANES)
+                    n_steps += walk_lanes<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
+                                                tmin, tmax, nb, b0, b1, W, win, verify, nk, nmin);
+                else
+                    n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
+                                                   inv_dt, tmin, tmax, nb, b0, b1, W, win, stage, lane, verify, nk,
+                                                   nmin);
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
@@ -591,10 +692,14 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
     }
 }
 
-template __global__ void radiation_kernel<0>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
-                                             unsigned long long*, int);
-template __global__ void radiation_kernel<1>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
-                                             unsigned long long*, int);
+template __global__ void radiation_kernel<0, false>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
+                                                    unsigned long long*, int);
+template __global__ void radiation_kernel<1, false>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
+                                                    unsigned long long*, int);
+template __global__ void radiation_kernel<0, true>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
+                                                   unsigned long long*, int);
+template __global__ void radiation_kernel<1, true>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
+                                                   unsigned long long*, int);
 
 // ---------------------------------------------------------------------------
 // Sum the chunk partials into the trace store, chunks in order.

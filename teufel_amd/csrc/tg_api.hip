@@ -47,6 +47,7 @@ struct LaunchPlan {
     size_t cursor_count = 0;
     int max_window = 0;
     int groups = 1;
+    bool lanes = true;  // per-lane walk (tg_tuning.walk == 2)
 };
 
 }  // namespace
@@ -78,6 +79,7 @@ struct tg_context {
     size_t cap_partial = 0;
     // walk statistics (diagnostic; on when TG_STATS=1 in the environment)
     bool stats_on = false;
+    int default_walk = 2;
     unsigned long long* d_stats = nullptr;
     size_t cap_stats = 0;
     long long last_steps = 0, last_walks = 0, last_items = 0;
@@ -290,11 +292,23 @@ size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(f
 
 // LDS of one wave: (window + 2 trash buckets) x 6 components x 64 lanes, plus
 // the FP64 kernel's record chunks.
-size_t stage_bytes(int prec) { return (prec == TG_FP64 ? SourceOf<0>::lds_doubles : SourceOf<1>::lds_doubles) * sizeof(double); }
-
-int max_window_for_lds(const tg_context* c, int prec)
+size_t stage_bytes(int prec, bool lanes)
 {
-    return (int)((c->max_lds - stage_bytes(prec)) / (6 * kLanes * cell_bytes(prec))) - 2;
+    if (lanes) return 0;  // the per-lane walk reads records straight from global memory
+    return (prec == TG_FP64 ? SourceOf<0>::lds_doubles : SourceOf<1>::lds_doubles) * sizeof(double);
+}
+
+int max_window_for_lds(const tg_context* c, int prec, bool lanes)
+{
+    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec))) - 2;
+}
+
+// tg_tuning.walk: 1 = wave-uniform walk, 2 = per-lane walk; 0 picks the
+// context default (2, or the TG_WALK environment variable at tg_create).
+bool lane_walk(const tg_context* c, const tg_tuning* tun)
+{
+    int w = (tun && tun->walk > 0) ? tun->walk : c->default_walk;
+    return w != 1;
 }
 
 // Build the work list for screens `ids`, particles [p0,p1).
@@ -325,7 +339,8 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     plan.n_chunks = n_chunks;
     plan.chunk = chunk;
     plan.screen_ids = ids;
-    int wmax_lds = max_window_for_lds(c, prec);
+    plan.lanes = lane_walk(c, tun);
+    int wmax_lds = max_window_for_lds(c, prec, plan.lanes);
     size_t part_off = 0;
     for (int id : ids) {
         ScreenHost& s = c->screens[id];
@@ -409,16 +424,24 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = (size_t)(plan.max_window + 2) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec);
+    size_t lds = (size_t)(plan.max_window + 2) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    if (prec == TG_FP64)
-        radiation_kernel<0><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
-            c->d_items, c->d_sdev, (const RecF64*)c->d_rec[0], c->n_points, c->d_cursors,
-            c->stats_on ? c->d_stats : nullptr, plan.max_window);
+    const unsigned grid = (unsigned)plan.items.size();
+    unsigned long long* st = c->stats_on ? c->d_stats : nullptr;
+    const RecF64* r64 = (const RecF64*)c->d_rec[0];
+    const RecMixed* rmx = (const RecMixed*)c->d_rec[1];
+    if (prec == TG_FP64 && plan.lanes)
+        radiation_kernel<0, true><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, r64, c->n_points,
+                                                                   c->d_cursors, st, plan.max_window);
+    else if (prec == TG_FP64)
+        radiation_kernel<0, false><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, r64, c->n_points,
+                                                                    c->d_cursors, st, plan.max_window);
+    else if (plan.lanes)
+        radiation_kernel<1, true><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, rmx, c->n_points,
+                                                                   c->d_cursors, st, plan.max_window);
     else
-        radiation_kernel<1><<<(unsigned)plan.items.size(), kLanes, lds, c->stream>>>(
-            c->d_items, c->d_sdev, (const RecMixed*)c->d_rec[1], c->n_points, c->d_cursors,
-            c->stats_on ? c->d_stats : nullptr, plan.max_window);
+        radiation_kernel<1, false><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, rmx, c->n_points,
+                                                                    c->d_cursors, st, plan.max_window);
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
@@ -459,8 +482,8 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         eff = c->tuned[prec];
     bool use_graph = eff.use_graph != 0;
     char keybuf[256];
-    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk, eff.window_groups,
-             screen_id, p0, p1, c->screens.size());
+    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk,
+             eff.window_groups, eff.walk, screen_id, p0, p1, c->screens.size());
     std::string key(keybuf);
     if (use_graph && c->gexec && c->graph_key == key) {
         TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -569,11 +592,13 @@ int tg_create(int device, tg_context** out)
     c->max_lds = prop.sharedMemPerBlock;
     const char* st = getenv("TG_STATS");
     c->stats_on = st && st[0] == '1';
-    if (hipFuncSetAttribute((const void*)radiation_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)c->max_lds) != hipSuccess ||
-        hipFuncSetAttribute((const void*)radiation_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)c->max_lds) != hipSuccess)
-        (void)hipGetLastError();
+    const char* wk = getenv("TG_WALK");
+    c->default_walk = (wk && wk[0] == '1') ? 1 : 2;
+    const void* kernels[] = {(const void*)radiation_kernel<0, false>, (const void*)radiation_kernel<1, false>,
+                             (const void*)radiation_kernel<0, true>, (const void*)radiation_kernel<1, true>};
+    for (const void* k : kernels)
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->max_lds) != hipSuccess)
+            (void)hipGetLastError();
     *out = c;
     return TG_OK;
 }
@@ -870,7 +895,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // Time each window size on a particle sample, per screen; keep the
     // fastest per screen (the launch shape adapts to each screen's bins).
     const int cands[] = {6, 8, 12, 16, 24};
-    int wmax = max_window_for_lds(c, precision);
+    int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     int sample = std::min(c->n_particles, std::max(64, c->n_particles / 32));
     std::vector<std::vector<double>> saved(c->screens.size());
     for (size_t id = 0; id < c->screens.size(); ++id) {
