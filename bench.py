@@ -169,6 +169,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the launch sequence from a hipGraph")
+    ap.add_argument("--groups", type=int, default=0, help="window groups per (tile, chunk); 0 = automatic")
+    ap.add_argument("--chunk", type=int, default=0, help="particles per chunk; 0 = automatic")
+    ap.add_argument("--walk", type=int, default=0, help="1 = wave-uniform walk, 2 = per-lane walk, 0 = default")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -186,8 +189,9 @@ def main():
     n_pairs = w.n_pairs()
     prec = TG_MIXED if args.precision == "mixed" else TG_FP64
     tuning = None
-    if args.window or args.graph:
-        tuning = {"bins_per_window": args.window, "use_graph": 1 if args.graph else 0}
+    if args.window or args.graph or args.groups or args.chunk or args.walk:
+        tuning = {"bins_per_window": args.window, "use_graph": 1 if args.graph else 0,
+                  "window_groups": args.groups, "particles_per_chunk": args.chunk, "walk": args.walk}
     tuned = None
     if not args.no_autotune and not args.window:
         t = time.time()

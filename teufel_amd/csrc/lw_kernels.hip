@@ -246,10 +246,11 @@ template <>
 struct RcpOf<float> { using type = RcpF32; };
 
 // LDS trace window of one wave: cell (bucket j, component c, lane) at
-// ((j*6)+c)*64 + lane, W buckets plus two per-lane trash buckets (W, W+1)
-// that absorb the absent targets of the branch-free common deposit.  Every
-// cell has exactly one writer (its own lane) and a wave's LDS accesses execute
-// in order, so plain read-add-write is race-free and deterministic.
+// ((j*6)+c)*64 + lane, W buckets plus one per-lane trash bucket (W) that
+// absorbs the absent targets of the branch-free common deposit (both targets
+// may land there; its contents are never read).  Every cell has exactly one
+// writer (its own lane) and a wave's LDS accesses execute in order, so plain
+// read-add-write is race-free and deterministic.
 template <class F>
 struct Window {
     F* lane_base;  // lds + lane
@@ -308,7 +309,7 @@ __device__ __forceinline__ void deposit_window(const Window<F>& win, int W, bool
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
     const TwoBuckets<F> t = step_common<F>(typename RcpOf<F>::type{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2,
-                                           t0, dt, inv_dt, n_bins, b0, b1, W, W + 1);
+                                           t0, dt, inv_dt, n_bins, b0, b1, W, W);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
@@ -615,8 +616,8 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
     }
     int2* __restrict__ cur_base = cursors + it.cursor_off;
     const Window<F> win{reinterpret_cast<F*>(lds) + lane};
-    // FP64: two record chunks after the window and its trash buckets
-    double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)(lds_window + 2) * 6 * kLanes);
+    // FP64 wave-uniform walk: two record chunks after the window and its trash bucket
+    double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)(lds_window + 1) * 6 * kLanes);
     unsigned long long n_steps = 0, n_walks = 0;
     // seed the cursors of a group that does not start at bucket 0
     if (it.bin_lo > 0) {

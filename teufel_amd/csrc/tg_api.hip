@@ -290,7 +290,7 @@ int default_window(const tg_context* c, int n_bins)
 
 size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
 
-// LDS of one wave: (window + 2 trash buckets) x 6 components x 64 lanes, plus
+// LDS of one wave: (window + 1 trash bucket) x 6 components x 64 lanes, plus
 // the FP64 kernel's record chunks.
 size_t stage_bytes(int prec, bool lanes)
 {
@@ -300,7 +300,7 @@ size_t stage_bytes(int prec, bool lanes)
 
 int max_window_for_lds(const tg_context* c, int prec, bool lanes)
 {
-    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec))) - 2;
+    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec))) - 1;
 }
 
 // tg_tuning.walk: 1 = wave-uniform walk, 2 = per-lane walk; 0 picks the
@@ -324,7 +324,7 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     if (tun && tun->particles_per_chunk > 0) {
         n_chunks = (np + tun->particles_per_chunk - 1) / tun->particles_per_chunk;
     } else {
-        long long target = 16LL * std::max(c->n_cu, 1);
+        long long target = 64LL * std::max(c->n_cu, 1);
         n_chunks = (int)std::max<long long>(1, (target + total_tiles - 1) / total_tiles);
         n_chunks = std::min(n_chunks, std::max(1, np / 32));
     }
@@ -364,11 +364,14 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     }
     plan.partial_doubles = part_off;
     // Window groups: each (tile, chunk) is split into groups of consecutive
-    // windows so that there are several times more work items than wave
-    // slots; a group seeds its cursors by search instead of walking from 0.
+    // windows so that the work items fill the resident wave slots (limited
+    // by the LDS window) about 32 times over, which keeps the tail of the
+    // launch short; a group seeds its cursors by search instead of walking
+    // from step 0.
     long long base_items = total_tiles * n_chunks;
-    long long slots = 8LL * std::max(c->n_cu, 1);
-    int groups = (int)std::max<long long>(1, std::min<long long>(16, (4 * slots + base_items - 1) / base_items));
+    const size_t lds_wave = (size_t)(plan.max_window + 1) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
+    long long slots = (long long)std::max(c->n_cu, 1) * (long long)std::max<size_t>(1, c->max_lds / lds_wave);
+    int groups = (int)std::max<long long>(1, std::min<long long>(64, (32 * slots + base_items - 1) / base_items));
     if (tun && tun->window_groups > 0) groups = tun->window_groups;
     plan.groups = groups;
     long long cur = 0;
@@ -424,7 +427,7 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = (size_t)(plan.max_window + 2) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
+    size_t lds = (size_t)(plan.max_window + 1) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
     const unsigned grid = (unsigned)plan.items.size();
     unsigned long long* st = c->stats_on ? c->d_stats : nullptr;
@@ -894,7 +897,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     }
     // Time each window size on a particle sample, per screen; keep the
     // fastest per screen (the launch shape adapts to each screen's bins).
-    const int cands[] = {6, 8, 12, 16, 24};
+    const int cands[] = {3, 4, 6, 8, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     int sample = std::min(c->n_particles, std::max(64, c->n_particles / 32));
     std::vector<std::vector<double>> saved(c->screens.size());
