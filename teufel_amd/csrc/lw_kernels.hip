@@ -537,11 +537,46 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
             }
         }
     }
-    PointT<F> p0, p1;
+    PointT<F> p0, p1, p2;
     eval_point<PREC>(rp[kl], Px, Py, Pz, t0, dt, inv_dt, p0);
     bool walking = active && kl < NP - 1 && p0.idx < b1;
-    Rec r1 = rp[kl + 1 < NP ? kl + 1 : NP - 1];
     int steps = 0;
+#if TG_WALK2
+    // Two steps per iteration: the two new points are independent, so their
+    // evaluations interleave (twice the instruction-level parallelism of one
+    // step); the deposits stay in step order.
+    Rec ra = rp[kl + 1 < NP ? kl + 1 : NP - 1];
+    Rec rb = rp[kl + 2 < NP ? kl + 2 : NP - 1];
+    while (__ballot(walking)) {
+        const Rec na = rp[kl + 3 < NP ? kl + 3 : NP - 1];
+        const Rec nb = rp[kl + 4 < NP ? kl + 4 : NP - 1];
+        eval_point<PREC>(ra, Px, Py, Pz, t0, dt, inv_dt, p1);
+        eval_point<PREC>(rb, Px, Py, Pz, t0, dt, inv_dt, p2);
+        deposit_window<F>(win, W, walking, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        const bool second = walking && p1.idx < b1 && kl + 1 < NP - 1;
+        deposit_window<F>(win, W, second, p1, p2, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        steps += 2;
+        if (walking) {
+            if (p1.idx >= b1) {
+                walking = false;            // crossing step (kl, kl+1)
+            } else if (!second) {
+                kl += 1;                    // ran off the trajectory's end
+                walking = false;
+            } else if (p2.idx >= b1) {
+                kl += 1;                    // crossing step (kl+1, kl+2)
+                p0 = p1;
+                walking = false;
+            } else {
+                kl += 2;
+                p0 = p2;
+                walking = kl < NP - 1;
+            }
+        }
+        ra = na;
+        rb = nb;
+    }
+#else
+    Rec r1 = rp[kl + 1 < NP ? kl + 1 : NP - 1];
     while (__ballot(walking)) {
         const Rec rn = rp[kl + 2 < NP ? kl + 2 : NP - 1];
         eval_point<PREC>(r1, Px, Py, Pz, t0, dt, inv_dt, p1);
@@ -558,6 +593,8 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
         }
         r1 = rn;
     }
+    (void)p2;
+#endif
     const bool stopped = active && kl < NP - 1;
     const int k_next = wave_min(stopped ? kl : INT_MAX);
     nk = k_next == INT_MAX ? NP - 1 : k_next;

@@ -35,7 +35,12 @@ struct ScreenHost {
     int n_tiles = 0;
     bool perm_valid = false;
     int window[2] = {0, 0};  // autotuned bins per window per precision (0 = default)
+    // per tile: bucket position of its first pixel at the profile steps
+    // (kProfile samples of the middle particle), for work estimates
+    std::vector<double> tile_prof;
 };
+
+constexpr int kProfile = 9;
 
 struct LaunchPlan {
     int n_chunks = 0;
@@ -249,7 +254,7 @@ void kd_split(std::vector<int>& idx, const std::vector<double>& feat, int nf, in
 int ensure_tiles(tg_context* c, ScreenHost& s)
 {
     if (s.perm_valid) return TG_OK;
-    const int nf = 9;
+    const int nf = kProfile;
     std::vector<double> samp(4 * nf, 0.0);
     if (c->n_particles > 0 && c->n_points > 0) {
         int p = c->n_particles / 2;
@@ -274,6 +279,9 @@ int ensure_tiles(tg_context* c, ScreenHost& s)
     s.n_tiles = (s.n_pix + kLanes - 1) / kLanes;
     std::vector<int> perm((size_t)s.n_tiles * kLanes, -1);
     for (int i = 0; i < s.n_pix; ++i) perm[i] = idx[i];
+    s.tile_prof.assign((size_t)s.n_tiles * nf, 0.0);
+    for (int t = 0; t < s.n_tiles; ++t)
+        for (int j = 0; j < nf; ++j) s.tile_prof[(size_t)t * nf + j] = feat[(size_t)perm[(size_t)t * kLanes] * nf + j];
     if (s.d_perm) TG_HIP(c, hipFree(s.d_perm));
     s.d_perm = nullptr;
     TG_HIP(c, hipMalloc((void**)&s.d_perm, perm.size() * sizeof(int)));
@@ -309,6 +317,27 @@ bool lane_walk(const tg_context* c, const tg_tuning* tun)
 {
     int w = (tun && tun->walk > 0) ? tun->walk : c->default_walk;
     return w != 1;
+}
+
+// Estimated trajectory steps of one particle whose arrival falls into
+// buckets [lo, hi) for a tile: its profile is piecewise linear between the
+// kProfile sample steps.
+double steps_in_buckets(const tg_context* c, const ScreenHost& s, int tile, int lo, int hi)
+{
+    const double* b = s.tile_prof.data() + (size_t)tile * kProfile;
+    double steps = 0.0;
+    for (int j = 0; j + 1 < kProfile; ++j) {
+        const double k0 = (double)(c->n_points - 1) * j / (kProfile - 1);
+        const double k1 = (double)(c->n_points - 1) * (j + 1) / (kProfile - 1);
+        const double a = std::min(b[j], b[j + 1]), e = std::max(b[j], b[j + 1]);
+        if (e > a) {
+            const double ov = std::min<double>(hi, e) - std::max<double>(lo, a);
+            if (ov > 0) steps += (k1 - k0) * ov / (e - a);
+        } else if (a >= lo && a < hi) {
+            steps += k1 - k0;
+        }
+    }
+    return steps;
 }
 
 // Build the work list for screens `ids`, particles [p0,p1).
@@ -401,6 +430,21 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
         }
     }
     plan.cursor_count = (size_t)cur;
+    // Dispatch the heaviest items first (longest-processing-time order), so
+    // that the launch does not end on a few long waves.  The order of the
+    // items changes no result: each writes its own partial-trace region.
+    std::vector<double> cost(plan.items.size());
+    for (size_t i = 0; i < plan.items.size(); ++i) {
+        const WorkItem& w = plan.items[i];
+        const ScreenHost& s = c->screens[ids[w.screen]];
+        cost[i] = (w.p1 - w.p0) * (1.0 + steps_in_buckets(c, s, w.tile, w.bin_lo, w.bin_hi));
+    }
+    std::vector<size_t> order(plan.items.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<WorkItem> sorted(plan.items.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = plan.items[order[i]];
+    plan.items.swap(sorted);
     return TG_OK;
 }
 
