@@ -22,7 +22,7 @@ roofline: the radiation kernel's algorithmic FP operations (106 per pair) per
 launch / its mean launch duration from HIP events on the launch stream,
 against the vector peak of the precision mix (VALU-bound: no MFMA shape).
 traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, read
-from the summary file profiles/traffic_<kernel>_<workload>.txt when present.
+from profiles/traffic_radiation_kernel_<precision>_<workload>.txt when present.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2]
        python bench.py --calibrate-copy BYTES   (a plain copy of BYTES, for
@@ -50,7 +50,8 @@ PEAK_FP64 = 78.6e12            # MI355X FP64 vector
 PEAK_FP32 = 157.3e12           # MI355X FP32 vector (published, packed)
 FP64_OPS_MIXED = 23.0          # of the 106, done in FP64 by the mixed kernel
 METRIC = "LW field evals/s (particle-step\u00d7pixel), mixed vs FP64 GPU at \u22641e-5 rel energy"
-KERNEL_NAME = {TG_MIXED: "radiation_kernel<1>", TG_FP64: "radiation_kernel<0>"}
+KERNEL_NAME = {TG_MIXED: "radiation_kernel<1, true>", TG_FP64: "radiation_kernel<0, true>"}
+PREC_NAME = {TG_MIXED: "mixed", TG_FP64: "fp64"}
 
 
 def mixed_peak():
@@ -142,11 +143,11 @@ def calibrate_copy(nbytes):
     log("copy kernel: read %d B, wrote %d B" % (4 * n, 4 * n))
 
 
-def read_traffic(kernel, workload):
-    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), from the committed summary."""
-    tag = kernel.replace("<", "").replace(">", "")
-    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.txt" % (tag, workload))
+def read_traffic(prec_name, workload):
+    """HBM bytes per launch of the radiation kernel measured by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), from the
+    committed summary profiles/traffic_radiation_kernel_<prec>_<workload>.txt."""
+    path = os.path.join(ROOT, "profiles", "traffic_radiation_kernel_%s_%s.txt" % (prec_name, workload))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -264,7 +265,7 @@ def main():
                    "bins": s0.n_bins, "screens": len(w.screens), "n_pairs": n_pairs,
                    "precision": args.precision, "tuning": tuned, "graph": bool(args.graph)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": read_traffic(KERNEL_NAME[prec], w.name),
+                     "frac": achieved / peak, "traffic": read_traffic(PREC_NAME[prec], w.name),
                      "kernel": KERNEL_NAME[prec], "flop_per_pair": FLOP_PER_PAIR, "kernel_ms": kern * 1e3},
         "cpu_baseline": cpu,
     }

@@ -23,4 +23,4 @@ def test_mixed_peak_between_fp64_and_fp32():
     sys.path.insert(0, ROOT)
     import bench
     assert bench.PEAK_FP64 < bench.mixed_peak() < bench.PEAK_FP32
-    assert bench.read_traffic("radiation_kernel<1>", "no-such-workload") is None
+    assert bench.read_traffic("mixed", "no-such-workload") is None
