@@ -104,7 +104,7 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
 template <class F>
 struct PointT {
     double ts;     // retarded (arrival) time
-    int idx;       // round((ts - t0)/dt)
+    int idx;       // bucket index round((ts - t0)/dt)
     F f[6];        // Ex, Ey, Ez, Bx, By, Bz
 };
 using Point = PointT<double>;
@@ -166,6 +166,8 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
 // Retarded time and bucket index of the mixed kernel, also returning 1/R:
 // 1/sqrt from the FP32 hardware estimate and one FP64 Newton step (~2^-46
 // relative).  Every FMA is explicit, so every caller gets the same doubles.
+// The index rounds half to even, which differs from the reference's
+// half-away rounding only for a retarded time exactly on a bucket edge.
 __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, double dz, double t0,
                                              double inv_dt, double& y, double& ts, int& idx)
 {
@@ -177,7 +179,7 @@ __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, dou
     y = __builtin_fma(0.5 * y, e, y);
     const double R = r2 * y;
     ts = __builtin_fma(R, kInvSpeedOfLight, t);
-    idx = round_index((ts - t0) * inv_dt);
+    idx = rint_index((ts - t0) * inv_dt);
 }
 
 // Mixed precision: FP64 distance, retarded time and the n_z beta_z part of
@@ -253,13 +255,14 @@ struct RcpOf<float> { using type = RcpF32; };
 // read-add-write is race-free and deterministic.
 template <class F>
 struct Window {
-    F* lane_base;  // lds + lane
-    __device__ __forceinline__ F* cell(int j) const { return lane_base + j * (6 * kLanes); }
+    using LdsF = __attribute__((address_space(3))) F;
+    LdsF* lane_base;  // lds + lane
+    __device__ __forceinline__ LdsF* cell(int j) const { return lane_base + j * (6 * kLanes); }
     // bucket ja += aa f1 + ba f2 and bucket jb += ab f1 + bb f2 (ja != jb)
     __device__ __forceinline__ void add2(int ja, F aa, F ba, int jb, F ab, F bb, const F* f1, const F* f2) const
     {
-        F* ca = cell(ja);
-        F* cb = cell(jb);
+        LdsF* ca = cell(ja);
+        LdsF* cb = cell(jb);
         F va[6], vb[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) va[c] = ca[c * kLanes];
@@ -281,7 +284,7 @@ struct WindowAcc {
     const F* f2;
     __device__ __forceinline__ void operator()(int j, double a, double b) const
     {
-        F* c0 = w.cell(j - b0);
+        auto* c0 = w.cell(j - b0);
         const F fa = (F)a, fb = (F)b;
 #pragma unroll
         for (int c = 0; c < 6; ++c) c0[c * kLanes] += fa * f1[c] + fb * f2[c];
@@ -652,7 +655,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         tmax = t0 + dt * ((double)nb - 0.5);
     }
     int2* __restrict__ cur_base = cursors + it.cursor_off;
-    const Window<F> win{reinterpret_cast<F*>(lds) + lane};
+    const Window<F> win{(typename Window<F>::LdsF*)(reinterpret_cast<F*>(lds)) + lane};
     // FP64 wave-uniform walk: two record chunks after the window and its trash bucket
     double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)(lds_window + 1) * 6 * kLanes);
     unsigned long long n_steps = 0, n_walks = 0;

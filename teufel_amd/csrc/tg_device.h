@@ -128,4 +128,13 @@ __device__ __forceinline__ int round_index(double q)
     return (int)r;
 }
 
+// Nearest integer (half to even) of q, clamped into int range: the mixed
+// kernel's bucket index.
+__device__ __forceinline__ int rint_index(double q)
+{
+    double r = __builtin_rint(q);
+    r = __builtin_fmin(__builtin_fmax(r, -1.0e9), 1.0e9);
+    return (int)r;
+}
+
 }  // namespace tg
