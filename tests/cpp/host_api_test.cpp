@@ -10,6 +10,16 @@
 //       orbit seen by a PointObserver<Bunch> equals mu0 I / (2R) within 1e-3.
 //       The orbit (one revolution, 200 steps) is the exact circle instead of
 //       the reference's Vay-tracked one.
+//   host_api_test h5info <screen.h5>
+//       reads a TEUFEL screen file with the HDF5 reader and prints its sizes,
+//       total area and the largest distance between an observation position
+//       and its triangle's centroid (no GPU needed)
+//   host_api_test diffraction <screen.h5> <trajectory> <out.dump> [fp64|mixed]
+//       tests/teufel.DiffractionScreen.cpp:130-160: the screen file read into a
+//       MeshedScreen<Bunch>, one macro-particle's trajectory (from
+//       <trajectory>, written by tests/test_host_api.py), integrate, print
+//       totalEnergy(); the Python test checks 0.463 +- 0.020 uJ (the reference
+//       test's own bound) and the traces against the oracle
 //   host_api_test screens <input> <mesh.dump> <screen.dump> [fp64|mixed]
 //       trajectories and a mesh definition from <input> (written by
 //       tests/test_host_api.py), integrated by MeshedScreen<Beam> (two
@@ -17,6 +27,7 @@
 //       the oracle by the Python test.  Prints the mesh's totalEnergy().
 //
 // Exit status: number of failed checks (0 = pass), 100+ on usage/IO errors.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -28,6 +39,7 @@
 #include "bunch.h"
 #include "global.h"
 #include "gpu_radiation.h"
+#include "h5lite.h"
 #include "screens.h"
 
 namespace {
@@ -251,6 +263,78 @@ int run_screens(const char* input, const char* mesh_out, const char* screen_out)
     return errors;
 }
 
+// ---- screen files ------------------------------------------------------------
+
+int run_h5info(const char* path)
+{
+    MeshedScreen<Bunch> mesh(path);
+    mesh.init();
+    H5LiteFile f(path);
+    const std::vector<double> cp = f.read_double("MeshCornerPoints");
+    const std::vector<int64_t> tri = f.read_int("MeshTriangles");
+    double worst = 0.0;
+    for (int ip = 0; ip < mesh.get_Np(); ip++) {
+        Vector c;
+        for (int k = 0; k < 3; k++) {
+            const int64_t v = tri[3 * ip + k];
+            c += Vector(cp[3 * v], cp[3 * v + 1], cp[3 * v + 2]) / 3.0;
+        }
+        worst = std::max(worst, (c - mesh.get_point(ip)).norm());
+    }
+    std::printf("Ncp %d Np %d Nt %d dt %.17g t0_0 %.17g area %.17g centroid_dev %.3g normal_0 %.6f %.6f %.6f\n",
+                mesh.get_Ncp(), mesh.get_Np(), mesh.get_Nt(), mesh.get_dt(), mesh.get_t0(0), mesh.get_total_area(),
+                worst, mesh.get_normal(0).x, mesh.get_normal(0).y, mesh.get_normal(0).z);
+    return 0;
+}
+
+int run_diffraction(const char* h5, const char* traj, const char* out)
+{
+    FILE* f = std::fopen(traj, "rb");
+    if (!f) return 101;
+    Reader r{f};
+    int32_t np = r.one<int32_t>(), npts = r.one<int32_t>();
+    if (!r.ok || np != 1 || npts < 2) {
+        std::fclose(f);
+        return 102;
+    }
+    std::vector<double> a(10 * (size_t)npts);
+    r.get(a.data(), a.size());
+    double q = r.one<double>();
+    std::fclose(f);
+    if (!r.ok) return 104;
+    ChargedParticle* e = new ChargedParticle(q, -q);
+    const size_t n = (size_t)npts;
+    for (size_t k = 0; k < n; k++) {
+        Vector X(a[n + k], a[2 * n + k], a[3 * n + k]), P(a[4 * n + k], a[5 * n + k], a[6 * n + k]),
+            A(a[7 * n + k], a[8 * n + k], a[9 * n + k]);
+        if (k == 0)
+            e->initTrajectory(a[k], X, P, A);
+        else
+            e->storeTrajectoryPoint(a[k], X, P, A);
+    }
+    Bunch bunch;
+    bunch.Add(e);
+    MeshedScreen<Bunch> screen(h5);
+    screen.init();
+    screen.zero();
+    screen.setSource(&bunch);
+    screen.integrate();
+    const double EuJ = screen.totalEnergy() * 1.0e6;
+    std::printf("radiation energy = %.17g uJ (reference test: 0.463 +- 0.020 uJ)\n", EuJ);
+    // the dump takes the screen's own file name; write it where asked
+    MeshedScreen<Bunch> copy(h5);
+    std::unique_ptr<double[]> buf(screen.getBuffer());
+    copy.fromBuffer(buf.get(), screen.getBufferSize());
+    std::vector<Vector> pts;
+    std::vector<FieldTrace*> tr;
+    for (int ip = 0; ip < copy.get_Np(); ip++) {
+        pts.push_back(copy.get_point(ip));
+        tr.push_back(copy.get_trace(ip));
+    }
+    writeDump(out, pts, tr);
+    return std::fabs(EuJ - 0.463) > 0.020;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -268,6 +352,11 @@ int main(int argc, char** argv)
         } else if (mode == "loop") {
             if (set_precision(argc > 2 ? argv[2] : nullptr)) return 100;
             errors = run_loop();
+        } else if (mode == "h5info" && argc >= 3) {
+            return run_h5info(argv[2]);
+        } else if (mode == "diffraction" && argc >= 5) {
+            if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
+            errors = run_diffraction(argv[2], argv[3], argv[4]);
         } else if (mode == "screens" && argc >= 5) {
             if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
             errors = run_screens(argv[2], argv[3], argv[4]);

@@ -1,13 +1,18 @@
 """The C++ host API (teufel_amd/host: ChargedParticle, Bunch, Beam, FieldTrace,
-MeshedScreen, ScreenObserver, PointObserver) driven by tests/cpp/host_api_test.cpp.
+MeshedScreen, ScreenObserver, PointObserver, the HDF5 screen reader) driven by
+tests/cpp/host_api_test.cpp.
 
-CPU: the host library and the test program build and link.
+CPU: the host library and the test program build and link; the HDF5 reader
+reads the reference's screen file (tests/golden/DiffractionScreen.h5, a copy
+of the data file of tests/teufel.DiffractionScreen.cpp).
 GPU: the program runs the reference's own deposit known answers
-(tests/teufel.integrate_field.cpp) and loop check (tests/teufel.loop.cpp)
-through the host classes, and integrates a MeshedScreen<Beam> and a
+(tests/teufel.integrate_field.cpp), loop check (tests/teufel.loop.cpp) and
+diffraction-screen energy (tests/teufel.DiffractionScreen.cpp, 0.463 +- 0.020
+uJ) through the host classes, and integrates a MeshedScreen<Beam> and a
 ScreenObserver<Bunch> whose dumps are compared with the oracle and the
 reference fixture (FP64: eps_E <= 1e-12; mixed: eps_W <= 1e-5).
 """
+import math
 import os
 import struct
 import subprocess
@@ -50,6 +55,69 @@ def test_host_library_builds_and_links():
     if not os.path.exists(os.path.join(ROOT, "teufel_amd", "lib", "libteufel_gpu.so")):
         B.build()
     assert os.path.exists(build_test_program())
+
+
+H5 = os.path.join(ROOT, "tests", "golden", "DiffractionScreen.h5")
+
+
+def test_screen_file_reader():
+    """The HDF5 reader on the reference's screen file (tests/DiffractionScreen.h5,
+    kept as a fixture): sizes and attributes as the file declares them, every
+    observation position at its triangle's centroid, a flat mesh facing -z."""
+    _program()
+    r = subprocess.run([BIN, "h5info", H5], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    v = r.stdout.split()
+    info = dict(zip(v[0::2], v[1::2]))
+    assert info["Ncp"] == "777" and info["Np"] == "1462" and info["Nt"] == "500"
+    assert float(info["dt"]) == 2e-13 and float(info["t0_0"]) == -5e-11
+    assert float(info["centroid_dev"]) < 1e-15
+    assert abs(float(info["area"]) - 1.23984687103686e-3) < 1e-15
+    assert r.stdout.rstrip().endswith("0.000000 0.000000 -1.000000")
+
+
+def diffraction_trajectory():
+    """tests/teufel.DiffractionScreen.cpp:73-103: 100 pC at gamma 20 on a
+    straight line from z = -10 m to +10 m (an empty lattice), passing z = 0 at
+    t = 0, 20000 steps."""
+    c = 2.99792458e8
+    gamma = 20.0
+    beta = math.sqrt(1.0 - 1.0 / (gamma * gamma))
+    bg = math.sqrt(gamma * gamma - 1.0)
+    n = 20000
+    t_start = -10.0 / (beta * c)
+    dt = (20.0 / (beta * c)) / n
+    k = np.arange(n + 1)
+    z = np.zeros((1, n + 1))
+    return oracle.Trajectories(t=(t_start + k * dt)[None], x=z, y=z, z=(-10.0 + beta * c * (k * dt))[None],
+                               ux=z, uy=z, uz=np.full((1, n + 1), bg), ax=z, ay=z, az=z,
+                               charge=np.array([-100e-12 / 1.6021766208e-19]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_diffraction_screen_end_to_end(tmp_path, precision):
+    tr = diffraction_trajectory()
+    traj = str(tmp_path / "traj.bin")
+    with open(traj, "wb") as f:
+        f.write(struct.pack("<ii", 1, tr.n_points))
+        for name in oracle.Trajectories.NAMES:
+            f.write(np.ascontiguousarray(getattr(tr, name), "<f8").tobytes())
+        f.write(np.ascontiguousarray(tr.charge, "<f8").tobytes())
+    dump = str(tmp_path / "diffraction.dump")
+    r = run("diffraction", H5, traj, dump, precision)
+    assert r.returncode == 0
+    energy = float(r.stdout.split("radiation energy =")[1].split()[0])
+    assert abs(energy - 0.463) <= 0.020          # the reference test's bound
+    pts, t0, dt, traces = read_dump(dump)
+    assert traces.shape == (1462, 500, 6) and dt == 2e-13
+    pick = np.arange(0, 1462, 7)
+    ref = oracle.integrate_screen(tr, pts[pick], t0[pick], dt, 500)
+    if precision == "fp64":
+        assert trace_rel_error(traces[pick], ref) <= 1e-12
+    else:
+        n = np.array([0.0, 0.0, -1.0])
+        assert energy_rel_error(pixel_energy(traces[pick], dt, n), pixel_energy(ref, dt, n))[0] <= 1e-5
 
 
 def pixel_energy(traces, dt, normal):

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "gpu_radiation.h"
+#include "h5lite.h"
 
 void writeDump(const std::string& filename, const std::vector<Vector>& points, const std::vector<FieldTrace*>& traces)
 {
@@ -70,6 +71,66 @@ MeshedScreen<SourceT>::MeshedScreen(const std::vector<Vector>& pts, const std::v
     if (t0.size() != pts.size()) throw IOexception("MeshedScreen - points/t0 size mismatch.");
     if (Nt < 0) throw IOexception("MeshedScreen - negative trace length.");
     for (int i = 0; i < Np; i++) A.push_back(new FieldTrace(t0[i], dt, (std::size_t)Nt));
+}
+
+template <typename SourceT>
+MeshedScreen<SourceT>::MeshedScreen(const std::string& filename) : Np(0), Nt(0), dt(0.0), FileName(filename)
+{
+    H5LiteFile f(filename);
+    const int64_t ncp = f.attr_int("MeshCornerPoints", "Ncp");
+    const std::vector<double> cp = f.read_double("MeshCornerPoints");
+    if (ncp < 0 || cp.size() != 3 * (size_t)ncp) throw IOexception("MeshedScreen - MeshCornerPoints size mismatch.");
+    for (int64_t i = 0; i < ncp; i++) corners.push_back(Vector(cp[3 * i], cp[3 * i + 1], cp[3 * i + 2]));
+    const int64_t np = f.attr_int("ObservationPosition", "Np");
+    const std::vector<double> op = f.read_double("ObservationPosition");
+    if (np < 0 || op.size() != 3 * (size_t)np) throw IOexception("MeshedScreen - ObservationPosition size mismatch.");
+    Np = (int)np;
+    for (int i = 0; i < Np; i++) points.push_back(Vector(op[3 * i], op[3 * i + 1], op[3 * i + 2]));
+    const std::vector<int64_t> tri = f.read_int("MeshTriangles");
+    if (tri.size() != 3 * (size_t)Np) throw IOexception("MeshedScreen - MeshTriangles size mismatch.");
+    for (int64_t v : tri) {
+        if (v < 0 || v >= ncp) throw IOexception("MeshedScreen - triangle corner index out of range.");
+        triangles.push_back((int)v);
+    }
+    std::vector<double> t0(Np, 0.0);
+    if (f.exists("ObservationTime")) {
+        const int64_t nt = f.attr_int("ObservationTime", "Nt");
+        if (nt < 0) throw IOexception("MeshedScreen - negative Nt.");
+        Nt = (int)nt;
+        dt = f.attr_double("ObservationTime", "dt");
+        t0 = f.read_double("ObservationTime");
+        if (t0.size() != (size_t)Np) throw IOexception("MeshedScreen - ObservationTime size mismatch.");
+    }
+    for (int i = 0; i < Np; i++) A.push_back(new FieldTrace(t0[i], dt, (std::size_t)Nt));
+    if (f.exists("ElMagField")) {
+        const std::vector<double> fld = f.read_double("ElMagField");
+        if (fld.size() != (size_t)Np * Nt * 6) throw IOexception("MeshedScreen - ElMagField size mismatch.");
+        for (int i = 0; i < Np; i++)
+            std::memcpy(A[i]->data(), &fld[(size_t)i * Nt * 6], (size_t)Nt * 6 * sizeof(double));
+    }
+}
+
+template <typename SourceT>
+void MeshedScreen<SourceT>::init()
+{
+    if (triangles.empty()) return;
+    area.assign(Np, 0.0);
+    normal.assign(Np, Vector());
+    for (int ip = 0; ip < Np; ip++) {
+        const Vector p1 = corners[triangles[3 * ip]], p2 = corners[triangles[3 * ip + 1]],
+                     p3 = corners[triangles[3 * ip + 2]];
+        Vector n = cross(p3 - p1, p2 - p1);
+        area[ip] = n.norm() * 0.5;
+        normal[ip] = n / n.norm();
+    }
+}
+
+template <typename SourceT>
+double MeshedScreen<SourceT>::get_total_area() const
+{
+    double total = 0.0;
+    for (double a : area) total += a;
+    return total;
 }
 
 template <typename SourceT>

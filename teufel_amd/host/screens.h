@@ -29,9 +29,21 @@ public:
                  const std::vector<double>& area = std::vector<double>(),
                  const std::vector<Vector>& normal = std::vector<Vector>(),
                  const std::string& filename = std::string());
+    // Read a TEUFEL screen file (HDF5, src/mesh.cpp:30-218): datasets
+    // MeshCornerPoints [Ncp][3], MeshTriangles [Np][3] (corner indices),
+    // ObservationPosition [Np][3], optional ObservationTime [Np] (attributes
+    // Nt, dt; without it the traces are empty) and ElMagField [Np][Nt][6].
+    explicit MeshedScreen(const std::string& filename);
     ~MeshedScreen() override;
 
-    void init() {}
+    // cell areas and unit normals n = (p3-p1) x (p2-p1) / |...| of the
+    // triangles of a screen read from file (src/mesh.cpp:227-247); a screen
+    // built from points keeps the areas and normals it was given
+    void init();
+    int get_Ncp() const { return (int)corners.size(); }
+    double get_area(int ip) const { return area.at(ip); }
+    Vector get_normal(int ip) const { return normal.at(ip); }
+    double get_total_area() const;
     void zero();
     int get_Np() const { return Np; }
     int get_Nt() const { return Nt; }
@@ -57,6 +69,8 @@ private:
     std::vector<Vector> normal;
     std::vector<FieldTrace*> A;
     std::string FileName;
+    std::vector<Vector> corners;             // MeshCornerPoints (file screens)
+    std::vector<int> triangles;              // MeshTriangles, 3 per cell
 };
 
 template <typename SourceT>
