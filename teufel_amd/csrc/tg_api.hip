@@ -939,33 +939,40 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         if (out) *out = best;
         return TG_OK;
     }
-    // Time each window size on a particle sample, per screen; keep the
-    // fastest per screen (the launch shape adapts to each screen's bins).
+    // Time each window size on a particle sample of all screens together, as
+    // they are launched: every wave of a launch gets the LDS of the largest
+    // window, so one window for the whole batch is what occupancy depends on.
     const int cands[] = {3, 4, 6, 8, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     int sample = std::min(c->n_particles, std::max(64, c->n_particles / 32));
+    int max_bins = 0;
     std::vector<std::vector<double>> saved(c->screens.size());
     for (size_t id = 0; id < c->screens.size(); ++id) {
         ScreenHost& s = c->screens[id];
+        max_bins = std::max(max_bins, s.n_bins);
         if (s.n_bins < 2 || s.n_pix == 0) continue;
         size_t nt = (size_t)s.n_pix * s.n_bins * 6;
         saved[id].resize(nt);
         TG_HIP(c, hipMemcpy(saved[id].data(), s.d_traces, nt * sizeof(double), hipMemcpyDeviceToHost));
-        float best_ms = 1e30f;
-        int best_w = default_window(c, s.n_bins);
-        for (int w : cands) {
-            if (w > wmax || w > s.n_bins) continue;
-            tg_tuning t = {};
-            t.bins_per_window = w;
-            int rc = integrate_impl(c, precision, &t, (int)id, 0, sample);
-            if (rc) return rc;
-            if (c->kernel_ms < best_ms) {
-                best_ms = c->kernel_ms;
-                best_w = w;
-            }
+    }
+    float best_ms = 1e30f;
+    int best_w = default_window(c, max_bins);
+    for (int w : cands) {
+        if (w > wmax || w > max_bins) continue;
+        tg_tuning t = {};
+        t.bins_per_window = w;
+        int rc = integrate_impl(c, precision, &t, -1, 0, sample);
+        if (rc) return rc;
+        if (c->kernel_ms < best_ms) {
+            best_ms = c->kernel_ms;
+            best_w = w;
         }
+    }
+    for (size_t id = 0; id < c->screens.size(); ++id) {
+        ScreenHost& s = c->screens[id];
         s.window[precision] = best_w;
-        TG_HIP(c, hipMemcpy(s.d_traces, saved[id].data(), nt * sizeof(double), hipMemcpyHostToDevice));
+        if (saved[id].empty()) continue;
+        TG_HIP(c, hipMemcpy(s.d_traces, saved[id].data(), saved[id].size() * sizeof(double), hipMemcpyHostToDevice));
     }
     best.bins_per_window = 0;  // per-screen windows stored on the screens
     c->tuned[precision] = best;
