@@ -134,6 +134,18 @@ int tg_screen_poynting(tg_context* ctx, int screen_id, double* out);
 int tg_deposit_only(tg_context* ctx, int n_pseudo, int n_points, const double* ts,
                     const double* fields, double t0, double dt, int n_bins, double* trace);
 
+/* The field at given times and points: ChargedParticle::RetardedField(double
+ * t, Vector x) (src/particle.cpp:612-769) summed over particles [p_begin,
+ * p_end) in order, as Bunch/Beam::RetardedField (src/bunch.cpp:729-737,
+ * src/beam.cpp:433-441) sum them; SnapshotObserver::integrate
+ * (src/snapshot.cpp:72-91) is one call with one time for all cells.
+ * obs_time[n_obs], obs_pos[n_obs][3], fields[n_obs][6] (host; accumulate = 1
+ * adds into fields).  A point before a particle's first trajectory point is
+ * back-extrapolated with its start velocity, one after its last sees nothing
+ * from it (the reference's rules).  TG_ERR_STATE without trajectories. */
+int tg_fields_at(tg_context* ctx, int n_obs, const double* obs_time, const double* obs_pos,
+                 int p_begin, int p_end, double* fields, int accumulate);
+
 /* Pick tuning for the context's current screens and trajectories by timing
  * candidate launch shapes on the device; the result is used by later
  * tg_integrate calls with tuning == NULL and written to `out` if non-NULL. */

@@ -53,6 +53,7 @@ def lib():
         L.or_track_undulator.argtypes = [i, _dp, _dp, d, d, d, d, i, d, d, i, _dp] + [_dp] * 10 + [i]
         L.or_retarded_field.argtypes = [ctypes.c_void_p, i, i, _dp, _dp]
         L.or_retarded_time.argtypes = [ctypes.c_void_p, i, i, _dp]
+        L.or_field_at.argtypes = [ctypes.c_void_p, i, d, _dp, _dp]
         _lib = L
     return _lib
 
@@ -152,6 +153,16 @@ def retarded_field(tr, p, k, obs):
     s = _or_struct(tr)
     obs = np.ascontiguousarray(obs, dtype=np.float64)
     lib().or_retarded_field(ctypes.byref(s), p, k, _ptr(obs), _ptr(out))
+    return out
+
+
+def field_at(tr, p, obs_time, obs):
+    """ChargedParticle::RetardedField(double, Vector) restated (oracle)."""
+    out = np.zeros(6)
+    s = _or_struct(tr)
+    obs = np.ascontiguousarray(obs, dtype=np.float64)
+    if lib().or_field_at(ctypes.byref(s), p, obs_time, _ptr(obs), _ptr(out)):
+        raise ValueError("empty trajectory")
     return out
 
 

@@ -104,6 +104,101 @@ void or_retarded_field(const or_trajectories* tr, int p, int k, const double obs
     f[3] = BField.x; f[4] = BField.y; f[5] = BField.z;
 }
 
+int or_field_at(const or_trajectories* tr, int p, double obs_time, const double obs[3], double f[6])
+{
+    /* ChargedParticle::RetardedField(double, Vector)  particle.cpp:612-769 */
+    const int NP = tr->n_points;
+    const size_t base = (size_t)p * (size_t)NP;
+    for (int i = 0; i < 6; i++) f[i] = 0.0;
+    if (NP < 1) return 1;                                       /* particle.cpp:615 RANGEexception */
+    vec3 O = v_make(obs[0], obs[1], obs[2]);
+    int i1 = 0;
+    double R1 = v_norm(v_sub(O, traj_x(tr, base + i1)));
+    double t1 = obs_time - (tr->t[base + i1] + R1 / OR_SPEED_OF_LIGHT);
+    int i2 = NP - 1;
+    double R2 = v_norm(v_sub(O, traj_x(tr, base + i2)));
+    double t2 = obs_time - (tr->t[base + i2] + R2 / OR_SPEED_OF_LIGHT);
+    if (!(t2 <= 0.0)) return 0;                                 /* particle.cpp:633: no signal yet */
+    vec3 SourceX, SourceBeta, SourceBetaPrime;
+    double gamma2;
+    if (t1 <= 0.0) {
+        /* particle.cpp:641-660: back-extrapolation with the start velocity */
+        vec3 Rc = v_div(v_sub(O, traj_x(tr, base)), OR_SPEED_OF_LIGHT);
+        SourceBeta = traj_u(tr, base);
+        double betagamma2 = v_abs2nd(SourceBeta);
+        gamma2 = betagamma2 + 1.0;
+        double gamma = sqrt(gamma2);
+        SourceBeta = v_div(SourceBeta, gamma);
+        double t0 = obs_time;
+        double bR = v_dot(SourceBeta, Rc);
+        double bb = v_dot(SourceBeta, SourceBeta);
+        double RR = v_dot(Rc, Rc);
+        double SourceT = (t0 - bR - sqrt((t0 - bR) * (t0 - bR) - (1 - bb) * (t0 * t0 - RR))) / (1 - bb);
+        SourceX = v_add(traj_x(tr, base), v_mul(v_mul(SourceBeta, OR_SPEED_OF_LIGHT), SourceT));
+        SourceBetaPrime = v_make(0.0, 0.0, 0.0);
+    } else {
+        /* particle.cpp:662-745: bisection to one step, then secant refinement */
+        while (i2 - i1 > 1) {
+            int iMid = (i2 + i1) / 2;
+            double RMid = v_norm(v_sub(O, traj_x(tr, base + iMid)));
+            double tMid = obs_time - (tr->t[base + iMid] + RMid / OR_SPEED_OF_LIGHT);
+            if (tMid > 0.0) {
+                i1 = iMid;
+                t1 = tMid;
+            } else {
+                i2 = iMid;
+                t2 = tMid;
+            }
+        }
+        double frac = t1 / (t1 - t2);
+        double T1 = tr->t[base + i1], T2 = tr->t[base + i2];
+        vec3 X1 = traj_x(tr, base + i1), X2 = traj_x(tr, base + i2);
+        double SourceT = T1 * (1.0 - frac) + T2 * frac;
+        SourceX = v_add(v_mul(X1, 1.0 - frac), v_mul(X2, frac));
+        double R = v_norm(v_sub(O, SourceX));
+        double dt = SourceT + R / OR_SPEED_OF_LIGHT - obs_time;
+        double r1 = v_norm(v_sub(O, X1));
+        double r2 = v_norm(v_sub(O, X2));
+        double dtdfrac = (T2 - T1) + (r2 - r1) / OR_SPEED_OF_LIGHT;
+        double dfrac = dt / dtdfrac;
+        int number_ref = 1;
+        while (fabs(dfrac) > 1e-5 && number_ref < 10) {
+            double Lastdt = dt;
+            frac -= dfrac;
+            SourceT = T1 * (1.0 - frac) + T2 * frac;
+            SourceX = v_add(v_mul(X1, 1.0 - frac), v_mul(X2, frac));
+            R = v_norm(v_sub(O, SourceX));
+            dt = SourceT + R / OR_SPEED_OF_LIGHT - obs_time;
+            dtdfrac = -(dt - Lastdt) / dfrac;
+            dfrac = dt / dtdfrac;
+            number_ref++;
+        }
+        frac -= dfrac;
+        SourceX = v_add(v_mul(X1, 1.0 - frac), v_mul(X2, frac));
+        SourceBeta = v_add(v_mul(traj_u(tr, base + i1), 1.0 - frac), v_mul(traj_u(tr, base + i2), frac));
+        double betagamma2 = v_abs2nd(SourceBeta);
+        gamma2 = betagamma2 + 1.0;
+        double gamma = sqrt(gamma2);
+        SourceBeta = v_div(SourceBeta, gamma);
+        SourceBetaPrime = v_add(v_mul(traj_a(tr, base + i1), 1.0 - frac), v_mul(traj_a(tr, base + i2), frac));
+        SourceBetaPrime = v_div(SourceBetaPrime, gamma);
+    }
+    /* particle.cpp:747-766 */
+    vec3 RVec = v_sub(O, SourceX);
+    double R = v_norm(RVec);
+    vec3 N = v_div(RVec, v_norm(RVec));
+    double scale = tr->charge[p] * OR_ELEMENTARY_CHARGE / (4.0 * OR_PI * OR_EPS0);
+    double bn3rd = pow(1.0 - v_dot(SourceBeta, N), 3.0);
+    vec3 NmB = v_sub(N, SourceBeta);
+    vec3 EField = v_div(NmB, R * R * bn3rd * gamma2);
+    EField = v_add(EField, v_div(v_cross(N, v_cross(NmB, SourceBetaPrime)), R * bn3rd * OR_SPEED_OF_LIGHT));
+    EField = v_mul(EField, scale);
+    vec3 BField = v_div(v_cross(N, EField), OR_SPEED_OF_LIGHT);
+    f[0] = EField.x; f[1] = EField.y; f[2] = EField.z;
+    f[3] = BField.x; f[4] = BField.y; f[5] = BField.z;
+    return 0;
+}
+
 int or_deposit_step(double ts1, double ts2, const double f1[6], const double f2[6],
                     double t0, double dt, int nots, double* trace)
 {

@@ -50,6 +50,14 @@ double or_retarded_time(const or_trajectories* tr, int p, int k, const double ob
  * f = (Ex,Ey,Ez,Bx,By,Bz). */
 void or_retarded_field(const or_trajectories* tr, int p, int k, const double obs[3], double f[6]);
 
+/* ChargedParticle::RetardedField(double, Vector)  src/particle.cpp:612-769:
+ * the field of particle p at obs at time obs_time, from the point on its
+ * trajectory on the observer's past light cone (bisection to one step,
+ * secant refinement, linear interpolation; back-extrapolation with the start
+ * velocity before the trajectory; zero after its end).  Returns 1 for an
+ * empty trajectory (the reference throws). */
+int or_field_at(const or_trajectories* tr, int p, double obs_time, const double obs[3], double f[6]);
+
 /* The per-step bucket deposit of ChargedParticle::integrateFieldTrace,
  * src/particle.cpp:788-865, for one trajectory step whose end points are
  * observed at ts1, ts2 with fields f1, f2.  trace = [n_bins][6]. */

@@ -7,9 +7,11 @@
 //       and the same 1e-6 tolerance
 //   host_api_test loop [fp64|mixed]
 //       tests/teufel.loop.cpp:160-190: the field at the centre of a circular
-//       orbit seen by a PointObserver<Bunch> equals mu0 I / (2R) within 1e-3.
-//       The orbit (one revolution, 200 steps) is the exact circle instead of
-//       the reference's Vay-tracked one.
+//       orbit seen by a PointObserver<Bunch> and by the snapshot
+//       Bunch::RetardedField(tau, centre) equals mu0 I / (2R) within 1e-3; a
+//       SnapshotObserver<Bunch> centred there gives the same field.  The orbit
+//       (one revolution, 200 steps) is the exact circle instead of the
+//       reference's Vay-tracked one.
 //   host_api_test h5info <screen.h5>
 //       reads a TEUFEL screen file with the HDF5 reader and prints its sizes,
 //       total area and the largest distance between an observation position
@@ -41,6 +43,7 @@
 #include "gpu_radiation.h"
 #include "h5lite.h"
 #include "screens.h"
+#include "snapshot.h"
 
 namespace {
 
@@ -136,6 +139,21 @@ int run_loop()
     int errors = std::fabs(b - expected) / expected > 1e-3;
     std::printf("loop: R = %.6g m, tau = %.6g s, |B| = %.6g T, expected %.6g T%s\n", radius, tau, b, expected,
                 errors ? "  <-- failed" : "");
+    // the snapshot of tests/teufel.loop.cpp:166-172: RetardedField(tau, centre)
+    ElMagField snap = bunch->RetardedField(tau, Vector(-radius, 0.0, 0.0));
+    if (std::fabs(snap.B().norm() - expected) / expected > 1e-3) {
+        std::printf("loop: snapshot |B| = %.6g T  <-- failed\n", snap.B().norm());
+        errors++;
+    } else {
+        std::printf("loop: snapshot |B| = %.6g T\n", snap.B().norm());
+    }
+    SnapshotObserver<Bunch> so("", Vector(-radius, 0.0, 0.0), Vector(1e-3, 0, 0), Vector(0, 1e-3, 0), 3, 3, tau);
+    so.setSource(bunch);
+    so.integrate();
+    if (so.getField(1, 1).B().x != snap.B().x || so.getField(1, 1).E().z != snap.E().z) {
+        std::printf("loop: SnapshotObserver centre cell differs from RetardedField\n");
+        errors++;
+    }
     // buffer exchange round trip (src/observer.h getBuffer / fromBuffer)
     std::size_t n = obs.getBufferSize();
     std::unique_ptr<double[]> buf(obs.getBuffer());

@@ -159,6 +159,32 @@ def test_padded_trajectories_add_zeros(ctx, prec):
         assert trace_rel_error(out, ref) <= 1e-4
 
 
+def test_fields_at_matches_oracle(ctx):
+    say("single-time field RetardedField(t, x) of a 4-electron bunch at 400 (t, x), all branches")
+    w = W.make("c2", n_particles=4, n_steps=600, seed=3)
+    u, b = w.und, w.bunch
+    tr = oracle.track_undulator(b.x0, b.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods, u.origin)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    rng = np.random.default_rng(21)
+    n = 400
+    pts = np.stack([rng.normal(size=n) * 0.05, rng.normal(size=n) * 0.05, rng.uniform(-1.0, 12.0, n)], axis=1)
+    times = rng.uniform(-2e-9, 4.5e-8, n)
+    got = ctx.fields_at(times, pts)
+    ref = np.zeros((n, 6))
+    for i in range(n):
+        for p in range(tr.n_particles):
+            ref[i] += oracle.field_at(tr, p, times[i], pts[i])
+    zero = ~ref.any(axis=1)
+    assert zero.any() and (~zero).any()
+    assert not got[zero].any()
+    assert trace_rel_error(got[~zero][:, None, :], ref[~zero][:, None, :]) <= FP64_TOL
+    # accumulate and particle ranges
+    half = ctx.fields_at(times, pts, p_range=(0, 2))
+    ctx.fields_at(times, pts, p_range=(2, 4), out=half, accumulate=True)
+    assert trace_rel_error(half[~zero][:, None, :], ref[~zero][:, None, :]) <= FP64_TOL
+
+
 def test_gpu_pusher_matches_oracle(ctx):
     say("device Vay pusher vs oracle pusher, 8 e- x 4000 steps")
     w = W.make("c2", n_particles=8, n_steps=4000, seed=5)

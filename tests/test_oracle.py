@@ -19,6 +19,12 @@ Each oracle function is checked against something that does not restate it:
   or_integrate_particle/_bunch/ the reference's traces (c1_ref, bunch_ref, loop_ref
   _screen                       fixtures, bitwise); loop |B| = mu0 I / (2R) (1e-3);
                                 undulator fundamental 2 gamma^2 c / (lambda_u (1+K^2/2))
+  or_field_at (single-time      Heaviside field of a uniformly moving charge at its
+  RetardedField)                present position, inside the trajectory (bisection,
+                                refinement) and before it (back-extrapolation); zero
+                                after the trajectory's end; Coulomb at rest; the loop
+                                snapshot |B| = mu0 I / (2R) (tests/teufel.loop.cpp:166-172);
+                                the compiled reference, bitwise
   or_screen_cells               integer-division centring of src/screen.cpp:98-110
   or_trace_poynting /           constant-field closed forms (sign and units)
   or_total_energy
@@ -233,6 +239,57 @@ def test_lienard_wiechert_high_precision():
         assert np.abs(f[3:] - B).max() <= 1e-12 * np.abs(B).max()
 
 
+# ---- single-time field (light-cone search) -------------------------------------------------
+
+def _straight(gamma=5.0, n=200, t_start=0.0, t_end=2e-9, q=-2.0, direction=(0.0, 0.0, 1.0)):
+    d = np.asarray(direction, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    bg = math.sqrt(gamma * gamma - 1.0)
+    beta = bg / gamma
+    t = np.linspace(t_start, t_end, n + 1)
+    X = np.outer(t, d) * beta * C
+    U = np.tile(d * bg, (n + 1, 1))
+    return _traj(t, X, U, np.zeros_like(U), q), beta * C * d
+
+
+@pytest.mark.parametrize("case", ["inside", "before"])
+def test_single_time_field_of_uniform_motion(case):
+    tr, v = _straight()
+    q = -2.0 * QE
+    for O in ([0.01, 0.0, 0.3], [0.0, -0.02, 0.1], [0.05, 0.03, -0.2]):
+        O = np.array(O)
+        if case == "inside":
+            T = 1.2e-9 + np.linalg.norm(O - np.array([0, 0, 0.2])) / C      # emission at t ~ 0.7 ns
+        else:
+            T = np.linalg.norm(O) / C * 0.5                               # emission before t = 0
+        f = oracle.field_at(tr, 0, T, O)
+        rp = O - v * T                        # present position of the charge at time T
+        b = np.linalg.norm(v) / C
+        r = np.linalg.norm(rp)
+        cos = rp @ v / (r * np.linalg.norm(v))
+        E = q * (1 - b * b) * rp / (4 * math.pi * EPS0 * r ** 3 * (1 - b * b * (1 - cos * cos)) ** 1.5)
+        assert np.abs(f[:3] - E).max() <= 1e-9 * np.abs(E).max(), (case, O)
+        assert np.abs(f[3:] - np.cross(v / C, E) / C).max() <= 1e-9 * np.abs(E).max() / C
+
+
+def test_single_time_field_after_trajectory_end_is_zero():
+    tr, _ = _straight()
+    assert not oracle.field_at(tr, 0, 2.5e-9 + 1.0 / C, np.array([0.0, 0.0, 1.0])).any()
+
+
+def test_single_time_field_at_rest_is_coulomb():
+    tr = _traj(np.array([0.0, 1e-9]), np.zeros((2, 3)), np.zeros((2, 3)), np.zeros((2, 3)), 1e-12 / QE)
+    f = oracle.field_at(tr, 0, 5e-10, np.array([0.0, 1e-3, 0.0]))
+    assert abs(f[1] / (1e-12 / (4 * math.pi * EPS0 * 1e-6)) - 1) < 1e-13
+
+
+def test_loop_snapshot_current_loop_field():
+    # tests/teufel.loop.cpp:166-172: RetardedField(tau, centre) of the loop electron
+    tr, a, w, tau = _circle()
+    f = oracle.field_at(tr, 0, tau, np.array([-a, 0.0, 0.0]))
+    assert abs(np.linalg.norm(f[3:]) / (MU0 * (1e-9 / tau) / (2 * a)) - 1) < 1e-3
+
+
 # ---- whole traces against the reference's fixtures --------------------------------------
 
 def test_c1_traces_equal_reference_fixture(golden):
@@ -377,6 +434,23 @@ def test_pseudo_deposit_equals_reference(seed):
     f = rng.normal(size=(n_p, n_pts, 6))
     t0, dt, n = rng.uniform(-0.2, 0.2), rng.uniform(0.03, 0.3), int(rng.integers(1, 40))
     assert np.array_equal(oracle.integrate_pseudo(ts, f, t0, dt, n), oracle.ref_pseudo_integrate(ts, f, t0, dt, n))
+
+
+@needs_ref
+def test_single_time_field_equals_reference():
+    w = W.make("c2", n_particles=4, n_steps=600, seed=3)
+    u, b = w.und, w.bunch
+    tr = oracle.track_undulator(b.x0, b.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods, u.origin)
+    rb = oracle.RefBeam(tr)
+    rng = np.random.default_rng(11)
+    try:
+        for _ in range(300):
+            p = int(rng.integers(0, 4))
+            O = np.array([rng.normal() * 0.05, rng.normal() * 0.05, rng.uniform(-1.0, 12.0)])
+            T = rng.uniform(-2e-9, 4.5e-8)
+            assert np.array_equal(oracle.field_at(tr, p, T, O), rb.retarded_field_at(p, T, O)), (p, T, O)
+    finally:
+        rb.close()
 
 
 @needs_ref

@@ -180,6 +180,20 @@ class RadiationContext:
         self._chk(self.lib.tg_screen_poynting(self.h, sid, _dptr(out)))
         return out
 
+    def fields_at(self, times, points, p_range=None, out=None, accumulate=False):
+        """Field (E, B) at points[i] at times[i] summed over the particles
+        (ChargedParticle/Bunch/Beam::RetardedField(t, x)); returns (n, 6)."""
+        points = _f64(points).reshape(-1, 3)
+        times = _f64(np.broadcast_to(np.asarray(times, dtype=np.float64), (points.shape[0],)))
+        n_particles, _ = self.trajectory_shape()
+        p0, p1 = p_range if p_range is not None else (0, n_particles)
+        if out is None:
+            out = np.zeros((points.shape[0], 6))
+            accumulate = False
+        self._chk(self.lib.tg_fields_at(self.h, points.shape[0], _dptr(times), _dptr(points), int(p0), int(p1),
+                                        _dptr(out), 1 if accumulate else 0))
+        return out
+
     def deposit_only(self, ts, fields, t0, dt, n_bins, trace=None):
         ts = _f64(ts)
         fields = _f64(fields)

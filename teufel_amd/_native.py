@@ -41,6 +41,7 @@ SIGNATURES = [
     ("tg_screen_traces_device", _i, [_vp, _i, ctypes.POINTER(_vp)]),
     ("tg_screen_poynting", _i, [_vp, _i, _dp]),
     ("tg_deposit_only", _i, [_vp, _i, _i, _dp, _dp, _d, _d, _i, _dp]),
+    ("tg_fields_at", _i, [_vp, _i, _dp, _dp, _i, _i, _dp, _i]),
     ("tg_autotune", _i, [_vp, _i, ctypes.POINTER(TgTuning)]),
     ("tg_last_timing", _i, [_vp, _fp, _fp]),
     ("tg_last_stats", _i, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 3),

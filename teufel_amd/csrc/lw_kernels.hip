@@ -802,6 +802,153 @@ __global__ void deposit_only_kernel(int n_pseudo, int n_points, const double* __
 }
 
 // ---------------------------------------------------------------------------
+// Field at given times and points: ChargedParticle::RetardedField(double,
+// Vector) (src/particle.cpp:612-769) summed over particles in order
+// (src/bunch.cpp:729-737, src/beam.cpp:433-441), one lane per observation.
+// The light-cone search (times, distances, bisection decisions, secant
+// refinement and its stopping test) is computed with correctly rounded
+// operations in the reference's order, so it takes the reference's path and
+// lands on the reference's interpolation fraction; the field amplitudes then
+// agree to rounding.
+// ---------------------------------------------------------------------------
+namespace snap {
+
+struct V {
+    double x, y, z;
+};
+__device__ __forceinline__ V mk(double x, double y, double z) { return V{x, y, z}; }
+
+__device__ __forceinline__ double div_rn(double a, double b) { return div_by_const(a, b, rcp_rn(b)); }
+
+// correctly rounded sqrt for any non-negative input
+__device__ __forceinline__ double sqrt_any(double x)
+{
+    return (x > 1e-300 && x < 1e300) ? sqrt_rn(x) : __builtin_sqrt(x);
+}
+
+struct Traj {
+    const double *t, *x, *y, *z, *ux, *uy, *uz, *ax, *ay, *az;
+};
+
+}  // namespace snap
+
+__global__ void field_at_kernel(int n_obs, const double* __restrict__ obs_time, const double* __restrict__ obs_pos,
+                                snap::Traj tr, const double* __restrict__ charge, int NP, int p0, int p1,
+                                double* __restrict__ out, int accumulate)
+{
+#pragma clang fp contract(off)
+    using snap::V;
+    using snap::mk;
+    using snap::div_rn;
+    using snap::sqrt_any;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_obs) return;
+    const double T = obs_time[i];
+    const V O = mk(obs_pos[3 * i], obs_pos[3 * i + 1], obs_pos[3 * i + 2]);
+    auto sub = [](V a, V b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); };
+    auto add = [](V a, V b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); };
+    auto mul = [](V a, double f) { return mk(a.x * f, a.y * f, a.z * f); };
+    auto divv = [](V a, double f) {
+        const double y = rcp_rn(f);
+        return mk(div_by_const(a.x, f, y), div_by_const(a.y, f, y), div_by_const(a.z, f, y));
+    };
+    auto norm = [&](V a) { return sqrt_any(a.x * a.x + a.y * a.y + a.z * a.z); };
+    auto dot = [](V a, V b) {
+        double t = a.x * b.x;
+        t += a.y * b.y;
+        t += a.z * b.z;
+        return t;
+    };
+    auto cross = [](V a, V b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); };
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = p0; p < p1; ++p) {
+        const size_t base = (size_t)p * NP;
+        auto X = [&](int k) { return mk(tr.x[base + k], tr.y[base + k], tr.z[base + k]); };
+        auto U = [&](int k) { return mk(tr.ux[base + k], tr.uy[base + k], tr.uz[base + k]); };
+        auto A = [&](int k) { return mk(tr.ax[base + k], tr.ay[base + k], tr.az[base + k]); };
+        auto tret = [&](int k) { return T - (tr.t[base + k] + div_by_const(norm(sub(O, X(k))), kSpeedOfLight, kInvSpeedOfLight)); };
+        int i1 = 0, i2 = NP - 1;
+        double t1 = tret(i1), t2 = tret(i2);
+        if (!(t2 <= 0.0)) continue;                              // particle.cpp:633: no signal yet
+        V SX, SB, SBP;
+        double gamma2;
+        if (t1 <= 0.0) {
+            // particle.cpp:641-660: back-extrapolation with the start velocity
+            const V Rc = divv(sub(O, X(0)), kSpeedOfLight);
+            SB = U(0);
+            gamma2 = (SB.x * SB.x + SB.y * SB.y + SB.z * SB.z) + 1.0;
+            const double gamma = sqrt_any(gamma2);
+            SB = divv(SB, gamma);
+            const double bR = dot(SB, Rc), bb = dot(SB, SB), RR = dot(Rc, Rc);
+            const double ST = div_rn(T - bR - sqrt_any((T - bR) * (T - bR) - (1 - bb) * (T * T - RR)), 1 - bb);
+            SX = add(X(0), mul(mul(SB, kSpeedOfLight), ST));
+            SBP = mk(0.0, 0.0, 0.0);
+        } else {
+            // particle.cpp:662-745: bisection to one step, secant refinement
+            while (i2 - i1 > 1) {
+                const int im = (i2 + i1) / 2;
+                const double tm = tret(im);
+                if (tm > 0.0) {
+                    i1 = im;
+                    t1 = tm;
+                } else {
+                    i2 = im;
+                    t2 = tm;
+                }
+            }
+            double frac = div_rn(t1, t1 - t2);
+            const double T1 = tr.t[base + i1], T2 = tr.t[base + i2];
+            const V X1 = X(i1), X2 = X(i2);
+            double ST = T1 * (1.0 - frac) + T2 * frac;
+            SX = add(mul(X1, 1.0 - frac), mul(X2, frac));
+            double R = norm(sub(O, SX));
+            double dtv = ST + div_by_const(R, kSpeedOfLight, kInvSpeedOfLight) - T;
+            const double r1 = norm(sub(O, X1)), r2 = norm(sub(O, X2));
+            double dtdfrac = (T2 - T1) + div_by_const(r2 - r1, kSpeedOfLight, kInvSpeedOfLight);
+            double dfrac = div_rn(dtv, dtdfrac);
+            int number_ref = 1;
+            while (__builtin_fabs(dfrac) > 1e-5 && number_ref < 10) {
+                const double last = dtv;
+                frac -= dfrac;
+                ST = T1 * (1.0 - frac) + T2 * frac;
+                SX = add(mul(X1, 1.0 - frac), mul(X2, frac));
+                R = norm(sub(O, SX));
+                dtv = ST + div_by_const(R, kSpeedOfLight, kInvSpeedOfLight) - T;
+                dtdfrac = div_rn(-(dtv - last), dfrac);
+                dfrac = div_rn(dtv, dtdfrac);
+                number_ref++;
+            }
+            frac -= dfrac;
+            SX = add(mul(X1, 1.0 - frac), mul(X2, frac));
+            SB = add(mul(U(i1), 1.0 - frac), mul(U(i2), frac));
+            gamma2 = (SB.x * SB.x + SB.y * SB.y + SB.z * SB.z) + 1.0;
+            const double gamma = sqrt_any(gamma2);
+            SB = divv(SB, gamma);
+            SBP = divv(add(mul(A(i1), 1.0 - frac), mul(A(i2), frac)), gamma);
+        }
+        // particle.cpp:747-766
+        const V RV = sub(O, SX);
+        const double R = norm(RV);
+        const V N = divv(RV, R);
+        const double scale = charge[p] * kElementaryCharge / (4.0 * kPi * kEpsNull);
+        const double k1 = 1.0 - dot(SB, N);
+        const double bn3 = k1 * k1 * k1;
+        const V NmB = sub(N, SB);
+        V E = divv(NmB, R * R * bn3 * gamma2);
+        E = add(E, divv(cross(N, cross(NmB, SBP)), R * bn3 * kSpeedOfLight));
+        E = mul(E, scale);
+        const V B = divv(cross(N, E), kSpeedOfLight);
+        acc[0] += E.x;
+        acc[1] += E.y;
+        acc[2] += E.z;
+        acc[3] += B.x;
+        acc[4] += B.y;
+        acc[5] += B.z;
+    }
+    for (int c = 0; c < 6; ++c) out[6 * (size_t)i + c] = accumulate ? out[6 * (size_t)i + c] + acc[c] : acc[c];
+}
+
+// ---------------------------------------------------------------------------
 // Per-pixel Poynting integral S = dt * sum E x B / mu0 (src/fieldtrace.cpp:192-198).
 // ---------------------------------------------------------------------------
 __global__ void poynting_kernel(const double* __restrict__ traces, int n_pix, int n_bins, double dt,

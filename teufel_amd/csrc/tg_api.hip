@@ -910,6 +910,49 @@ int tg_deposit_only(tg_context* c, int n_pseudo, int n_points, const double* ts,
     return TG_OK;
 }
 
+int tg_fields_at(tg_context* c, int n_obs, const double* obs_time, const double* obs_pos, int p_begin, int p_end,
+                 double* fields, int accumulate)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (n_obs < 0) return fail(c, TG_ERR_ARG, "tg_fields_at: negative n_obs");
+    if (n_obs > 0 && (!obs_time || !obs_pos || !fields)) return fail(c, TG_ERR_ARG, "tg_fields_at: null array");
+    if (!c->traj_set || c->n_points < 1)
+        return fail(c, TG_ERR_STATE, "tg_fields_at: no trajectory (RetardedField needs at least one point)");
+    if (p_begin < 0 || p_end > c->n_particles || p_begin > p_end) return fail(c, TG_ERR_RANGE, "particle range");
+    if (n_obs == 0) return TG_OK;
+    if (!finite_all(obs_time, (size_t)n_obs) || !finite_all(obs_pos, 3 * (size_t)n_obs))
+        return fail(c, TG_ERR_ARG, "tg_fields_at: non-finite observation");
+    TG_HIP(c, hipSetDevice(c->device));
+    double *d_t = nullptr, *d_x = nullptr, *d_f = nullptr;
+    auto cleanup = [&]() {
+        if (d_t) (void)hipFree(d_t);
+        if (d_x) (void)hipFree(d_x);
+        if (d_f) (void)hipFree(d_f);
+    };
+    const size_t n = (size_t)n_obs;
+    if (hipMalloc((void**)&d_t, n * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&d_x, 3 * n * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&d_f, 6 * n * sizeof(double)) != hipSuccess) {
+        cleanup();
+        return fail(c, TG_ERR_OOM, "tg_fields_at: device allocation failed");
+    }
+    hipError_t e = hipMemcpy(d_t, obs_time, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_x, obs_pos, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && accumulate) e = hipMemcpy(d_f, fields, 6 * n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        snap::Traj tr{raw(c, 0), raw(c, 1), raw(c, 2), raw(c, 3), raw(c, 4),
+                      raw(c, 5), raw(c, 6), raw(c, 7), raw(c, 8), raw(c, 9)};
+        field_at_kernel<<<(unsigned)((n + 63) / 64), 64, 0, c->stream>>>(
+            n_obs, d_t, d_x, tr, raw_charge(c), c->n_points, p_begin, p_end, d_f, accumulate);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(fields, d_f, 6 * n * sizeof(double), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_fields_at: ") + hipGetErrorString(e));
+    return TG_OK;
+}
+
 int tg_last_timing(tg_context* c, float* kernel_ms, float* total_ms)
 {
     if (check_ctx(c)) return TG_ERR_ARG;

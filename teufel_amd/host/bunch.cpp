@@ -48,3 +48,19 @@ void Beam::integrateFieldTrace(Vector ObservationPoint, FieldTrace* trace)
     std::vector<FieldTrace*> tr(1, trace);
     teufel_gpu::integrate(all, pos, tr);
 }
+
+ElMagField Bunch::RetardedField(double obs_time, Vector obs_pos)
+{
+    std::vector<ElMagField> out;
+    teufel_gpu::fields_at(P, std::vector<double>(1, obs_time), std::vector<Vector>(1, obs_pos), out);
+    return out[0];
+}
+
+ElMagField Beam::RetardedField(double obs_time, Vector obs_pos)
+{
+    std::vector<ChargedParticle*> all;
+    collect(all);
+    std::vector<ElMagField> out;
+    teufel_gpu::fields_at(all, std::vector<double>(1, obs_time), std::vector<Vector>(1, obs_pos), out);
+    return out[0];
+}

@@ -17,6 +17,8 @@ public:
     int getNOP() const { return (int)P.size(); }
     ChargedParticle* getParticle(int i);
     void integrateFieldTrace(Vector ObservationPoint, FieldTrace* trace);
+    // sum of the particles' RetardedField(t, x) (src/bunch.cpp:729-737)
+    ElMagField RetardedField(double obs_time, Vector obs_pos);
     // all particles in summation order
     void collect(std::vector<ChargedParticle*>& out) const { out.insert(out.end(), P.begin(), P.end()); }
 
@@ -33,6 +35,8 @@ public:
     Bunch* getBunch(int i);
     int getNOP() const;
     void integrateFieldTrace(Vector ObservationPoint, FieldTrace* trace);
+    // sum over bunches of their RetardedField(t, x) (src/beam.cpp:433-441)
+    ElMagField RetardedField(double obs_time, Vector obs_pos);
     void collect(std::vector<ChargedParticle*>& out) const
     {
         for (const Bunch* b : B) b->collect(out);

@@ -166,6 +166,31 @@ void integrate(const std::vector<ChargedParticle*>& particles, const std::vector
     check(ctx, tg_remove_screens(ctx), "tg_remove_screens");
 }
 
+void fields_at(const std::vector<ChargedParticle*>& particles, const std::vector<double>& times,
+               const std::vector<Vector>& points, std::vector<ElMagField>& out)
+{
+    if (times.size() != points.size()) throw IOexception("teufel_gpu::fields_at - times/points size mismatch.");
+    for (const ChargedParticle* p : particles)
+        if (p->getNP() < 1)
+            throw RANGEexception("ChargedParticle::RetardedField() - must initalize trajectory before calling");
+    out.assign(points.size(), ElMagField());
+    if (particles.empty() || points.empty()) return;
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    tg_context* ctx = context(s);
+    upload(s, particles);
+    std::vector<double> pos(3 * points.size()), f(6 * points.size(), 0.0);
+    for (size_t i = 0; i < points.size(); i++) {
+        pos[3 * i] = points[i].x;
+        pos[3 * i + 1] = points[i].y;
+        pos[3 * i + 2] = points[i].z;
+    }
+    check(ctx, tg_fields_at(ctx, (int)points.size(), times.data(), pos.data(), 0, (int)particles.size(), f.data(), 0),
+          "tg_fields_at");
+    for (size_t i = 0; i < points.size(); i++)
+        out[i] = ElMagField(Vector(f[6 * i], f[6 * i + 1], f[6 * i + 2]), Vector(f[6 * i + 3], f[6 * i + 4], f[6 * i + 5]));
+}
+
 void deposit_only(int n_pseudo, int n_points, const double* ts, const double* fields, FieldTrace* trace)
 {
     State& s = state();

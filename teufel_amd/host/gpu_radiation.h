@@ -30,6 +30,13 @@ void set_device(int device);
 void integrate(const std::vector<ChargedParticle*>& particles, const std::vector<Vector>& points,
                const std::vector<FieldTrace*>& traces);
 
+// The field of `particles` (summed in this order) at points[i] at times[i]
+// (RetardedField(double, Vector), src/particle.cpp:612-769), one device call
+// for all observations; out is resized to points.size().  A particle without
+// trajectory points raises RANGEexception, as the reference does.
+void fields_at(const std::vector<ChargedParticle*>& particles, const std::vector<double>& times,
+               const std::vector<Vector>& points, std::vector<ElMagField>& out);
+
 // The device bucket deposit driven by injected retarded times ts[n_pseudo][n_points]
 // and fields fields[n_pseudo][n_points][6], accumulated into trace.
 void deposit_only(int n_pseudo, int n_points, const double* ts, const double* fields, FieldTrace* trace);

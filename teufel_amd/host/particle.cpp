@@ -76,3 +76,11 @@ void ChargedParticle::integrateFieldTrace(Vector ObservationPoint, FieldTrace* t
     std::vector<FieldTrace*> tr(1, trace);
     teufel_gpu::integrate(one, pos, tr);
 }
+
+ElMagField ChargedParticle::RetardedField(double obs_time, Vector obs_pos)
+{
+    std::vector<ElMagField> out;
+    teufel_gpu::fields_at(std::vector<ChargedParticle*>(1, this), std::vector<double>(1, obs_time),
+                          std::vector<Vector>(1, obs_pos), out);
+    return out[0];
+}

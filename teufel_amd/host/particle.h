@@ -41,6 +41,11 @@ public:
     // Accumulate the field observed at ObservationPoint into trace.
     void integrateFieldTrace(Vector ObservationPoint, FieldTrace* trace);
 
+    // The field at obs_pos at time obs_time from the trajectory point on the
+    // observer's past light cone (src/particle.cpp:612-769), on the GPU.
+    // Throws RANGEexception without trajectory points.
+    ElMagField RetardedField(double obs_time, Vector obs_pos);
+
     // Incremented by every change of the trajectory (the device cache key).
     unsigned long long version() const { return Version; }
 
