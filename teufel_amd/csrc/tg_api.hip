@@ -357,10 +357,11 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
         n_chunks = (int)std::max<long long>(1, (target + total_tiles - 1) / total_tiles);
         n_chunks = std::min(n_chunks, std::max(1, np / 32));
     }
-    // cap the partial-trace scratch at 48 GB
+    // cap the partial-trace scratch at 16 GB (work items get finer through
+    // window groups, which cost no memory)
     size_t per_chunk = 0;
     for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
-    const size_t cap = (size_t)48 << 30;
+    const size_t cap = (size_t)16 << 30;
     while (n_chunks > 1 && (size_t)n_chunks * per_chunk * sizeof(double) > cap) --n_chunks;
     n_chunks = std::max(1, std::min(n_chunks, np));
     int chunk = (np + n_chunks - 1) / n_chunks;
