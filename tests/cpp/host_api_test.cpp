@@ -16,6 +16,9 @@
 //       reads a TEUFEL screen file with the HDF5 reader and prints its sizes,
 //       total area and the largest distance between an observation position
 //       and its triangle's centroid (no GPU needed)
+//   host_api_test h5write <screen.h5> <out.h5>
+//       reads a screen file, sets known trace values, writes it with
+//       MeshedScreen::writeFile and reads it back (no GPU needed)
 //   host_api_test diffraction <screen.h5> <trajectory> <out.dump> [fp64|mixed]
 //       tests/teufel.DiffractionScreen.cpp:130-160: the screen file read into a
 //       MeshedScreen<Bunch>, one macro-particle's trajectory (from
@@ -299,10 +302,45 @@ int run_h5info(const char* path)
         }
         worst = std::max(worst, (c - mesh.get_point(ip)).norm());
     }
+    double t0sum = 0.0;
+    for (int ip = 0; ip < mesh.get_Np(); ip++) t0sum += mesh.get_t0(ip);
+    const size_t last = cp.size() - 3;
+    std::printf("t0sum %.17g cp_last %.17g %.17g %.17g tri_last %lld\n", t0sum, cp[last], cp[last + 1], cp[last + 2],
+                (long long)tri.back());
     std::printf("Ncp %d Np %d Nt %d dt %.17g t0_0 %.17g area %.17g centroid_dev %.3g normal_0 %.6f %.6f %.6f\n",
                 mesh.get_Ncp(), mesh.get_Np(), mesh.get_Nt(), mesh.get_dt(), mesh.get_t0(0), mesh.get_total_area(),
                 worst, mesh.get_normal(0).x, mesh.get_normal(0).y, mesh.get_normal(0).z);
     return 0;
+}
+
+// read a screen file, give every trace sample a known value, write it with
+// writeFile and read it back
+int run_h5write(const char* in, const char* out)
+{
+    MeshedScreen<Bunch> a(in);
+    for (int ip = 0; ip < a.get_Np(); ip++) {
+        double* d = a.get_trace(ip)->data();
+        for (int k = 0; k < 6 * a.get_Nt(); k++) d[k] = ip + 1e-3 * k;
+    }
+    a.writeFile(out);
+    MeshedScreen<Bunch> b(out);
+    int errors = 0;
+    if (b.get_Np() != a.get_Np() || b.get_Nt() != a.get_Nt() || b.get_dt() != a.get_dt() || b.get_Ncp() != a.get_Ncp())
+        errors++;
+    for (int ip = 0; ip < a.get_Np() && !errors; ip++) {
+        if (!(b.get_point(ip).x == a.get_point(ip).x && b.get_t0(ip) == a.get_t0(ip))) errors++;
+        const double *x = a.get_trace(ip)->data(), *y = b.get_trace(ip)->data();
+        for (int k = 0; k < 6 * a.get_Nt(); k++)
+            if (x[k] != y[k]) {
+                errors++;
+                break;
+            }
+    }
+    a.init();
+    b.init();
+    if (a.get_total_area() != b.get_total_area()) errors++;
+    std::printf("h5write: %d points x %d samples written and read back, errors %d\n", b.get_Np(), b.get_Nt(), errors);
+    return errors;
 }
 
 int run_diffraction(const char* h5, const char* traj, const char* out)
@@ -372,6 +410,8 @@ int main(int argc, char** argv)
             errors = run_loop();
         } else if (mode == "h5info" && argc >= 3) {
             return run_h5info(argv[2]);
+        } else if (mode == "h5write" && argc >= 4) {
+            return run_h5write(argv[2], argv[3]);
         } else if (mode == "diffraction" && argc >= 5) {
             if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
             errors = run_diffraction(argv[2], argv[3], argv[4]);

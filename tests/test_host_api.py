@@ -14,6 +14,8 @@ reference fixture (FP64: eps_E <= 1e-12; mixed: eps_W <= 1e-5).
 """
 import math
 import os
+import re
+import shutil
 import struct
 import subprocess
 
@@ -60,20 +62,75 @@ def test_host_library_builds_and_links():
 H5 = os.path.join(ROOT, "tests", "golden", "DiffractionScreen.h5")
 
 
+def _h5dump():
+    for cand in (shutil.which("h5dump"), "/opt/conda/bin/h5dump"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def _h5dump_values(path, obj, kind="-d", extra=()):
+    """Numbers of a dataset (kind "-d") or attribute ("-a") printed by the HDF5
+    library's own h5dump (an independent reader of the same file)."""
+    r = subprocess.run([_h5dump(), "-m", "%.17g", kind, obj] + list(extra) + [path], capture_output=True, text=True,
+                       timeout=120, cwd="/tmp")
+    assert r.returncode == 0, r.stderr
+    body = r.stdout.split("DATA {", 1)[1].split("}", 1)[0]
+    body = re.sub(r"\([0-9,]+\):", " ", body)
+    return np.array([float(x) for x in body.replace(",", " ").split()])
+
+
 def test_screen_file_reader():
     """The HDF5 reader on the reference's screen file (tests/DiffractionScreen.h5,
     kept as a fixture): sizes and attributes as the file declares them, every
-    observation position at its triangle's centroid, a flat mesh facing -z."""
+    observation position at its triangle's centroid, a flat mesh facing -z,
+    and the values the HDF5 library's h5dump reads from the same file."""
     _program()
     r = subprocess.run([BIN, "h5info", H5], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     v = r.stdout.split()
     info = dict(zip(v[0::2], v[1::2]))
+    if _h5dump():
+        t0 = _h5dump_values(H5, "/ObservationTime")
+        total = 0.0
+        for x in t0:
+            total += x
+        assert float(info["t0sum"]) == total
+        cp = _h5dump_values(H5, "/MeshCornerPoints")
+        tail = r.stdout.split("cp_last")[1].split()[:3]
+        assert [float(x) for x in tail] == list(cp[-3:])
+        tri = _h5dump_values(H5, "/MeshTriangles")
+        assert float(info["tri_last"]) == tri[-1]
     assert info["Ncp"] == "777" and info["Np"] == "1462" and info["Nt"] == "500"
     assert float(info["dt"]) == 2e-13 and float(info["t0_0"]) == -5e-11
     assert float(info["centroid_dev"]) < 1e-15
     assert abs(float(info["area"]) - 1.23984687103686e-3) < 1e-15
     assert r.stdout.rstrip().endswith("0.000000 0.000000 -1.000000")
+
+
+def test_screen_file_writer(tmp_path):
+    """MeshedScreen::writeFile (src/mesh.cpp:427-736 layout): the written file
+    reads back identically, and the HDF5 library's h5dump reads its
+    structure, attributes and field values."""
+    _program()
+    out = str(tmp_path / "written.h5")
+    r = subprocess.run([BIN, "h5write", H5, out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    if not _h5dump():
+        pytest.skip("h5dump not available for the independent check")
+    hdr = subprocess.run([_h5dump(), "-H", out], capture_output=True, text=True, timeout=60, cwd="/tmp")
+    assert hdr.returncode == 0, hdr.stderr
+    for name, shape in (("ElMagField", "( 1462, 500, 6 )"), ("MeshCornerPoints", "( 777, 3 )"),
+                        ("MeshTriangles", "( 1462, 3 )"), ("ObservationPosition", "( 1462, 3 )"),
+                        ("ObservationTime", "( 1462 )")):
+        block = hdr.stdout.split('DATASET "%s"' % name)[1].split("DATASET")[0]
+        assert shape in block
+    assert _h5dump_values(out, "/ObservationTime/Nt", "-a").tolist() == [500]
+    assert _h5dump_values(out, "/ObservationTime/dt", "-a").tolist() == [2e-13]
+    assert _h5dump_values(out, "/MeshCornerPoints/Ncp", "-a").tolist() == [777]
+    f = _h5dump_values(out, "/ElMagField", "-d", ["-s", "3,1,0", "-c", "1,1,6"])
+    assert np.array_equal(f, 3 + 1e-3 * np.arange(6, 12))
+    assert np.array_equal(_h5dump_values(out, "/ObservationTime"), _h5dump_values(H5, "/ObservationTime"))
 
 
 def diffraction_trajectory():

@@ -6,6 +6,7 @@
 // messages.
 #include "h5lite.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -374,4 +375,233 @@ int64_t H5LiteFile::attr_int(const std::string& dataset, const std::string& name
     if (it == ds.attrs.end()) fail(dataset + ": no attribute " + name);
     if (!numeric(it->second.type) || it->second.count < 1) fail(dataset + "/" + name + ": not numeric");
     return element_int(it->second.type, it->second.data.data());
+}
+
+// ---- writer -------------------------------------------------------------------
+
+namespace {
+struct Out {
+    std::vector<uint8_t> b;
+    void at(uint64_t off, uint64_t v, int bytes)
+    {
+        if (b.size() < off + bytes) b.resize(off + bytes, 0);
+        for (int i = 0; i < bytes; ++i) b[off + i] = (uint8_t)(v >> (8 * i));
+    }
+    void raw(uint64_t off, const void* p, size_t n)
+    {
+        if (b.size() < off + n) b.resize(off + n, 0);
+        std::memcpy(&b[off], p, n);
+    }
+};
+
+// datatype message bodies: IEEE binary64 and signed 32-bit, little endian
+const uint8_t kTypeF64[24] = {0x11, 0x20, 0x3f, 0x00, 8, 0, 0, 0, 0, 0, 64, 0, 52, 11, 0, 52, 0xff, 0x03, 0, 0, 0, 0, 0, 0};
+const uint8_t kTypeI32[16] = {0x10, 0x08, 0x00, 0x00, 4, 0, 0, 0, 0, 0, 32, 0, 0, 0, 0, 0};
+constexpr uint64_t kLeafK = 4, kInternalK = 16;
+}  // namespace
+
+H5LiteWriter::Set& H5LiteWriter::find(const std::string& name)
+{
+    for (Set& s : sets)
+        if (s.name == name) return s;
+    throw IOexception("h5lite: no dataset " + name + " to attach an attribute to");
+}
+
+void H5LiteWriter::add_double(const std::string& name, const std::vector<uint64_t>& dims, const std::vector<double>& data)
+{
+    uint64_t n = 1;
+    for (uint64_t d : dims) n *= d;
+    if (n != data.size()) throw IOexception("h5lite: " + name + ": data size does not match its dimensions");
+    Set s{name, dims, true, std::vector<uint8_t>(data.size() * 8), {}};
+    if (!data.empty()) std::memcpy(s.bytes.data(), data.data(), s.bytes.size());
+    sets.push_back(s);
+}
+
+void H5LiteWriter::add_int32(const std::string& name, const std::vector<uint64_t>& dims, const std::vector<int32_t>& data)
+{
+    uint64_t n = 1;
+    for (uint64_t d : dims) n *= d;
+    if (n != data.size()) throw IOexception("h5lite: " + name + ": data size does not match its dimensions");
+    Set s{name, dims, false, std::vector<uint8_t>(data.size() * 4), {}};
+    if (!data.empty()) std::memcpy(s.bytes.data(), data.data(), s.bytes.size());
+    sets.push_back(s);
+}
+
+void H5LiteWriter::attr_int32(const std::string& dataset, const std::string& name, int32_t value)
+{
+    find(dataset).attrs.push_back(Attr{name, false, 0.0, value});
+}
+
+void H5LiteWriter::attr_double(const std::string& dataset, const std::string& name, double value)
+{
+    find(dataset).attrs.push_back(Attr{name, true, value, 0});
+}
+
+void H5LiteWriter::write(const std::string& filename) const
+{
+    if (sets.size() > 2 * kLeafK) throw IOexception("h5lite: at most 8 datasets per file");
+    std::vector<const Set*> order;
+    for (const Set& s : sets) order.push_back(&s);
+    std::sort(order.begin(), order.end(), [](const Set* a, const Set* b) { return a->name < b->name; });
+    for (size_t i = 1; i < order.size(); ++i)
+        if (order[i]->name == order[i - 1]->name) throw IOexception("h5lite: duplicate dataset " + order[i]->name);
+
+    // local heap data: "" at 0, then the names, each padded to 8 bytes
+    std::vector<uint8_t> heap(8, 0);
+    std::vector<uint64_t> name_off;
+    for (const Set* s : order) {
+        name_off.push_back(heap.size());
+        heap.insert(heap.end(), s->name.begin(), s->name.end());
+        heap.resize(pad8(heap.size() + 1), 0);
+    }
+    // object header message bodies of each dataset
+    auto dataspace = [](const std::vector<uint64_t>& dims) {
+        std::vector<uint8_t> m(8 + 8 * dims.size(), 0);
+        m[0] = 1;
+        m[1] = (uint8_t)dims.size();
+        for (size_t i = 0; i < dims.size(); ++i)
+            for (int k = 0; k < 8; ++k) m[8 + 8 * i + k] = (uint8_t)(dims[i] >> (8 * k));
+        return m;
+    };
+    struct Msg {
+        uint16_t type;
+        uint8_t flags;
+        std::vector<uint8_t> body;
+    };
+    std::vector<std::vector<Msg>> msgs(order.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+        const Set* s = order[i];
+        msgs[i].push_back({0x0001, 0, dataspace(s->dims)});
+        msgs[i].push_back(s->is_double ? Msg{0x0003, 1, std::vector<uint8_t>(kTypeF64, kTypeF64 + 20)}
+                                       : Msg{0x0003, 1, std::vector<uint8_t>(kTypeI32, kTypeI32 + 12)});
+        msgs[i].push_back({0x0008, 0, std::vector<uint8_t>(18, 0)});  // layout, addresses patched below
+        for (const Attr& a : s->attrs) {
+            std::vector<uint8_t> m(8, 0);
+            const size_t nlen = a.name.size() + 1;
+            const size_t tlen = a.is_double ? 20 : 12;
+            m[0] = 1;
+            m[2] = (uint8_t)nlen;
+            m[3] = (uint8_t)(nlen >> 8);
+            m[4] = (uint8_t)tlen;
+            m[6] = 8;  // scalar dataspace message
+            std::vector<uint8_t> name(pad8(nlen), 0);
+            std::memcpy(name.data(), a.name.c_str(), a.name.size());
+            m.insert(m.end(), name.begin(), name.end());
+            std::vector<uint8_t> t(a.is_double ? kTypeF64 : kTypeI32, (a.is_double ? kTypeF64 : kTypeI32) + tlen);
+            t.resize(pad8(tlen), 0);
+            m.insert(m.end(), t.begin(), t.end());
+            std::vector<uint8_t> sp(8, 0);
+            sp[0] = 1;
+            m.insert(m.end(), sp.begin(), sp.end());
+            uint8_t v[8] = {0};
+            if (a.is_double)
+                std::memcpy(v, &a.d, 8);
+            else
+                std::memcpy(v, &a.i, 4);
+            m.insert(m.end(), v, v + (a.is_double ? 8 : 4));
+            msgs[i].push_back({0x000C, 0, m});
+        }
+    }
+    auto header_size = [&](const std::vector<Msg>& ms) {
+        uint64_t n = 0;
+        for (const Msg& m : ms) n += 8 + pad8(m.body.size());
+        return n;
+    };
+    // addresses
+    const uint64_t root_oh = 96;
+    const uint64_t root_oh_size = 16 + 8 + 16;
+    const uint64_t heap_hdr = pad8(root_oh + root_oh_size);
+    const uint64_t heap_data = heap_hdr + 32;
+    const uint64_t btree = pad8(heap_data + heap.size());
+    const uint64_t btree_size = 24 + (2 * kInternalK + 1) * 8 + 2 * kInternalK * 8;
+    const uint64_t snod = pad8(btree + btree_size);
+    const uint64_t snod_size = 8 + 2 * kLeafK * 40;
+    std::vector<uint64_t> oh(order.size()), data(order.size());
+    uint64_t p = pad8(snod + snod_size);
+    for (size_t i = 0; i < order.size(); ++i) {
+        oh[i] = p;
+        p = pad8(p + 16 + header_size(msgs[i]));
+    }
+    for (size_t i = 0; i < order.size(); ++i) {
+        data[i] = p;
+        p = pad8(p + order[i]->bytes.size());
+    }
+    const uint64_t eof = p;
+
+    Out o;
+    o.raw(0, kSignature, 8);
+    o.at(13, 8, 1);
+    o.at(14, 8, 1);
+    o.at(16, kLeafK, 2);
+    o.at(18, kInternalK, 2);
+    o.at(24, 0, 8);
+    o.at(32, kUndefined, 8);
+    o.at(40, eof, 8);
+    o.at(48, kUndefined, 8);
+    o.at(56, 0, 8);          // root entry: link name offset
+    o.at(64, root_oh, 8);
+    o.at(72, 1, 4);          // cache type 1: B-tree and heap in the scratch pad
+    o.at(80, btree, 8);
+    o.at(88, heap_hdr, 8);
+    // root object header: one symbol table message
+    o.at(root_oh, 1, 1);
+    o.at(root_oh + 2, 1, 2);
+    o.at(root_oh + 4, 1, 4);
+    o.at(root_oh + 8, 8 + 16, 4);
+    o.at(root_oh + 16, 0x0011, 2);
+    o.at(root_oh + 18, 16, 2);
+    o.at(root_oh + 24, btree, 8);
+    o.at(root_oh + 32, heap_hdr, 8);
+    // local heap
+    o.raw(heap_hdr, "HEAP", 4);
+    o.at(heap_hdr + 8, heap.size(), 8);
+    o.at(heap_hdr + 16, 1, 8);  // free-list head: 1 is the "no free block" marker
+    o.at(heap_hdr + 24, heap_data, 8);
+    o.raw(heap_data, heap.data(), heap.size());
+    // group B-tree: one child, the symbol node
+    o.raw(btree, "TREE", 4);
+    o.at(btree + 6, order.empty() ? 0 : 1, 2);
+    o.at(btree + 8, kUndefined, 8);
+    o.at(btree + 16, kUndefined, 8);
+    o.at(btree + 24, 0, 8);
+    o.at(btree + 32, snod, 8);
+    o.at(btree + 40, order.empty() ? 0 : name_off.back(), 8);
+    o.at(btree + btree_size - 1, 0, 1);
+    // symbol table node
+    o.raw(snod, "SNOD", 4);
+    o.at(snod + 4, 1, 1);
+    o.at(snod + 6, order.size(), 2);
+    for (size_t i = 0; i < order.size(); ++i) {
+        o.at(snod + 8 + 40 * i, name_off[i], 8);
+        o.at(snod + 16 + 40 * i, oh[i], 8);
+    }
+    o.at(snod + snod_size - 1, 0, 1);
+    // dataset object headers and data
+    for (size_t i = 0; i < order.size(); ++i) {
+        std::vector<Msg>& ms = msgs[i];
+        std::vector<uint8_t>& lay = ms[2].body;
+        lay[0] = 3;
+        lay[1] = 1;
+        for (int k = 0; k < 8; ++k) lay[2 + k] = (uint8_t)(data[i] >> (8 * k));
+        for (int k = 0; k < 8; ++k) lay[10 + k] = (uint8_t)((uint64_t)order[i]->bytes.size() >> (8 * k));
+        o.at(oh[i], 1, 1);
+        o.at(oh[i] + 2, ms.size(), 2);
+        o.at(oh[i] + 4, 1, 4);
+        o.at(oh[i] + 8, header_size(ms), 4);
+        uint64_t q = oh[i] + 16;
+        for (const Msg& m : ms) {
+            o.at(q, m.type, 2);
+            o.at(q + 2, pad8(m.body.size()), 2);
+            o.at(q + 4, m.flags, 1);
+            o.raw(q + 8, m.body.data(), m.body.size());
+            q += 8 + pad8(m.body.size());
+        }
+        if (!order[i]->bytes.empty()) o.raw(data[i], order[i]->bytes.data(), order[i]->bytes.size());
+    }
+    o.at(eof - 1, o.b.size() >= eof ? o.b[eof - 1] : 0, 1);
+
+    FILE* f = std::fopen(filename.c_str(), "wb");
+    if (!f) throw IOexception("h5lite: cannot create " + filename);
+    const bool ok = std::fwrite(o.b.data(), 1, o.b.size(), f) == o.b.size();
+    if (std::fclose(f) != 0 || !ok) throw IOexception("h5lite: write error on " + filename);
 }

@@ -1,4 +1,4 @@
-// h5lite.h - a minimal read-only HDF5 reader for TEUFEL's screen files.
+// h5lite.h - a minimal HDF5 reader and writer for TEUFEL's screen files.
 //
 // TEUFEL stores meshed screens in HDF5 (src/mesh.cpp:30-218 reads them with
 // libhdf5: datasets MeshCornerPoints, MeshTriangles, ObservationPosition,
@@ -8,6 +8,9 @@
 // headers, contiguous or compact layouts, little-endian integer and IEEE
 // float data, version 1-3 attributes) that such files use, without linking
 // libhdf5.  Anything outside that subset raises IOexception naming it.
+// H5LiteWriter writes files of the same subset (superblock version 0, one
+// root group of up to eight contiguous datasets with scalar attributes), the
+// layout MeshedScreen::writeFile produces with libhdf5 (src/mesh.cpp:427-736).
 #pragma once
 
 #include <cstdint>
@@ -62,3 +65,32 @@ private:
     void need(uint64_t off, uint64_t n) const;
     uint64_t u(uint64_t off, int bytes) const;
 };
+
+class H5LiteWriter {
+public:
+    // dims row-major (slowest first); data has prod(dims) elements
+    void add_double(const std::string& name, const std::vector<uint64_t>& dims, const std::vector<double>& data);
+    void add_int32(const std::string& name, const std::vector<uint64_t>& dims, const std::vector<int32_t>& data);
+    // scalar attributes of an added dataset
+    void attr_int32(const std::string& dataset, const std::string& name, int32_t value);
+    void attr_double(const std::string& dataset, const std::string& name, double value);
+    void write(const std::string& filename) const;
+
+private:
+    struct Attr {
+        std::string name;
+        bool is_double;
+        double d;
+        int32_t i;
+    };
+    struct Set {
+        std::string name;
+        std::vector<uint64_t> dims;
+        bool is_double;
+        std::vector<uint8_t> bytes;
+        std::vector<Attr> attrs;
+    };
+    std::vector<Set> sets;
+    Set& find(const std::string& name);
+};
+

@@ -126,6 +126,41 @@ void MeshedScreen<SourceT>::init()
 }
 
 template <typename SourceT>
+void MeshedScreen<SourceT>::writeFile(const std::string& filename) const
+{
+    if (triangles.empty()) throw IOexception("MeshedScreen::writeFile - the screen has no mesh (not read from a file).");
+    H5LiteWriter w;
+    std::vector<double> cp, op, t0, fld;
+    for (const Vector& v : corners) cp.insert(cp.end(), {v.x, v.y, v.z});
+    for (const Vector& v : points) op.insert(op.end(), {v.x, v.y, v.z});
+    for (const FieldTrace* t : A) t0.push_back(t->get_t0());
+    w.add_double("MeshCornerPoints", {corners.size(), 3}, cp);
+    w.attr_int32("MeshCornerPoints", "Ncp", (int32_t)corners.size());
+    w.add_double("ObservationPosition", {(uint64_t)Np, 3}, op);
+    w.attr_int32("ObservationPosition", "Np", Np);
+    w.add_int32("MeshTriangles", {(uint64_t)Np, 3}, std::vector<int32_t>(triangles.begin(), triangles.end()));
+    w.attr_int32("MeshTriangles", "Ntri", Np);
+    w.add_double("ObservationTime", {(uint64_t)Np}, t0);
+    w.attr_int32("ObservationTime", "Nt", Nt);
+    w.attr_double("ObservationTime", "dt", dt);
+    if (Nt > 0) {
+        fld.reserve((size_t)Np * Nt * 6);
+        for (const FieldTrace* t : A) fld.insert(fld.end(), t->data(), t->data() + 6 * (size_t)Nt);
+        w.add_double("ElMagField", {(uint64_t)Np, (uint64_t)Nt, 6}, fld);
+    }
+    w.write(filename);
+}
+
+template <typename SourceT>
+void MeshedScreen<SourceT>::generateOutput()
+{
+    if (!triangles.empty())
+        writeFile(FileName);
+    else
+        writeDump(FileName, points, A);
+}
+
+template <typename SourceT>
 double MeshedScreen<SourceT>::get_total_area() const
 {
     double total = 0.0;

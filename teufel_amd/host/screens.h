@@ -59,7 +59,13 @@ public:
     std::size_t getBufferSize() override { return (std::size_t)Np * Nt * 6; }
     double* getBuffer() override;
     void fromBuffer(double* buffer, std::size_t size) override;
-    void generateOutput() override { writeDump(FileName, points, A); }
+    // HDF5 file with the datasets the file constructor reads plus ElMagField
+    // [Np][Nt][6] (src/mesh.cpp:427-736); needs the mesh triangles, so only
+    // for screens read from a file
+    void writeFile(const std::string& filename) const;
+    // writeFile(filename of the constructor) for screens read from a file
+    // (src/mesh.cpp:823-826), the plain dump otherwise
+    void generateOutput() override;
 
 private:
     int Np, Nt;
