@@ -67,7 +67,7 @@ def build_workload(ctx, name, n_particles=None):
     u = w.und
     t = time.time()
     ctx.track_undulator(w.bunch.x0, w.bunch.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods, u.origin,
-                        charge=w.bunch.charge)
+                        charge=w.bunch.charge, mass=w.bunch.mass)
     log("workload %s: %d particles x %d points tracked on device in %.2f s"
         % (name, w.n_particles, w.n_points, time.time() - t))
     ids = [ctx.add_screen(s.obs, s.t0, s.dt, s.n_bins) for s in w.screens]

@@ -85,6 +85,13 @@ class Bunch:
     charge: float = -1.0    # per particle, units of e
     gamma0: float = 60.0
 
+    @property
+    def mass(self):
+        """Per particle, units of m_e: a macro-particle of N electrons has
+        charge -N and mass N, keeping the electron's charge-to-mass ratio (as
+        tests/teufel.DiffractionScreen.cpp:78 builds one)."""
+        return abs(self.charge)
+
 
 def single_electron(gamma=60.0):
     bg = math.sqrt(gamma * gamma - 1.0)
