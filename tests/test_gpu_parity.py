@@ -137,6 +137,31 @@ def test_mixed_energy_error_and_determinism(ctx, walk):
     assert epsW <= MIXED_ENERGY_TOL
 
 
+@pytest.mark.parametrize("prec,walk", [(TG_FP64, 2), (TG_MIXED, 2), (TG_FP64, 1), (TG_MIXED, 1)])
+def test_persistent_waves_and_lane_offsets_bitwise(ctx, prec, walk, monkeypatch):
+    say("few persistent waves (many items each), per-lane offsets off: bitwise equal; prec=%d walk=%d"
+        % (prec, walk))
+    tr, obs, t0, dt, _ = _small_bunch()
+    tun = {"bins_per_window": 4, "particles_per_chunk": 7, "window_groups": 3, "walk": walk}
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 256)
+    ctx.integrate(prec, tuning=tun)
+    ref = ctx.read_traces(sid)
+    if prec == TG_FP64:
+        assert trace_rel_error(ref, oracle.integrate_screen(tr, obs, t0, dt, 256)) <= FP64_TOL
+    for env in ({"TG_SLOTS": "3"}, {"TG_LANE_CURSORS": "0"}, {"TG_SLOTS": "1", "TG_LANE_CURSORS": "0"}):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        with teufel_amd.RadiationContext(0) as c2:
+            c2.set_trajectories(tr)
+            s2 = c2.add_screen(obs, t0, dt, 256)
+            c2.integrate(prec, tuning=tun)
+            assert np.array_equal(c2.read_traces(s2), ref), env
+        for k in env:
+            monkeypatch.delenv(k)
+
+
 @pytest.mark.parametrize("prec", [TG_FP64, TG_MIXED])
 def test_padded_trajectories_add_zeros(ctx, prec):
     say("particles padded by repeating their last point (zero-length steps), prec=%d" % prec)

@@ -502,33 +502,50 @@ __device__ __forceinline__ int point_index(const typename RecOf<PREC>::type* __r
 }
 
 // Walk of one particle for the window [b0, b1) in which every lane (pixel)
-// follows its own step: from the wave-uniform start k each lane first skips,
-// with the retarded time alone, the steps that end before b0 (they deposit
-// nothing in the window), then evaluates and deposits its steps until one
-// ends at or beyond b1 (that crossing step is deposited here and again in the
-// next window).  Records are read per lane, one step ahead.  Lanes thus spend
-// full evaluations only on their own steps; the wave runs as long as its
-// longest lane.  Returns the main-loop trip count; nk = the smallest stopping
-// step over the lanes (the next window's start), nmin = the smallest bucket
-// at those steps.  With verify, k is a searched guess: step back until no
-// lane starts at or beyond b0 at point k.
+// follows its own step: from its start each lane first skips, with the
+// retarded time alone, the steps that end before b0 (they deposit nothing in
+// the window), then evaluates and deposits its steps until one ends at or
+// beyond b1 (that crossing step is deposited here and again in the next
+// window).  Records are read per lane, one step ahead.  Lanes thus spend full
+// evaluations only on their own steps; the wave runs as long as its longest
+// lane.  Returns the main-loop trip count; nk = the smallest stopping step
+// over the lanes (the next window's start), nmin = the smallest bucket at
+// those steps.  A lane starts at the wave-uniform k, or, with search, at the
+// point before the first point (from 1 on) whose bucket is >= b0, found by a
+// binary search over the retarded time (monotone in k; the skip from 0 would
+// stop at the same step), or, with lc_in (not null), at k plus its offset
+// from k to the step it stopped at in the previous window, saturated at 255
+// (every point between lies before b0, so the skip would have passed it).
+// lc_out (when not null) receives the offsets of this walk's stopping steps
+// from nk.
 template <int PREC>
 __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __restrict__ rp, int k, int NP,
                                           bool active, double Px, double Py, double Pz, double t0, double dt,
                                           double inv_dt, double tmin, double tmax, int n_bins, int b0, int b1,
-                                          int W, const Window<typename RecOf<PREC>::amp>& win, bool verify,
+                                          int W, const Window<typename RecOf<PREC>::amp>& win, bool search,
+                                          const unsigned char* __restrict__ lc_in, unsigned char* __restrict__ lc_out,
                                           int& nk, int& nmin)
 {
     using F = typename RecOf<PREC>::amp;
     using Rec = typename RecOf<PREC>::type;
-    if (verify) {
-        int back = 32;
-        while (k > 0 && __ballot(active && point_index<PREC>(rp, k, Px, Py, Pz, t0, dt, inv_dt) >= b0)) {
-            k = k > back ? k - back : 0;
-            back *= 2;
-        }
-    }
     int kl = k;
+    if (search) {
+        int lo = 1, hi = NP;  // first point in [lo, hi) with bucket >= b0; NP: none
+        bool open = active && lo < hi;
+        while (__ballot(open)) {
+            if (open) {
+                const int mid = (lo + hi) >> 1;
+                if (point_index<PREC>(rp, mid, Px, Py, Pz, t0, dt, inv_dt) >= b0)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+                open = lo < hi;
+            }
+        }
+        kl = min(lo - 1, NP - 1);
+    } else if (lc_in) {
+        kl = min(k + (int)lc_in[threadIdx.x], NP - 1);
+    }
     bool skip = active && kl < NP - 1;
     while (__ballot(skip)) {
         if (skip) {
@@ -602,6 +619,7 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     const int k_next = wave_min(stopped ? kl : INT_MAX);
     nk = k_next == INT_MAX ? NP - 1 : k_next;
     nmin = wave_min(stopped ? p0.idx : INT_MAX);
+    if (lc_out) lc_out[threadIdx.x] = (unsigned char)min(max(kl - nk, 0), 255);
     return steps;
 }
 
@@ -628,18 +646,19 @@ __device__ __forceinline__ int search_first_point(const typename RecOf<PREC>::ty
     return lo;
 }
 
+// One work item (tile, particle chunk, window group) on one wave.  cur_base
+// holds the item's cursors (p1-p0 int2), lane_cur (when not null) its
+// per-lane offsets (kLanes bytes per particle); both are private to the wave.
 template <int PREC, bool LANES>
-__global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items,
-                                                        const ScreenDev* __restrict__ screens,
-                                                        const typename RecOf<PREC>::type* __restrict__ recs,
-                                                        int n_points, int2* __restrict__ cursors,
-                                                        unsigned long long* __restrict__ stats, int lds_window)
+__device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __restrict__ screens,
+                                         const typename RecOf<PREC>::type* __restrict__ recs, int n_points,
+                                         int2* __restrict__ cur_base, unsigned char* __restrict__ lane_cur,
+                                         int lds_window, unsigned long long& n_steps, unsigned long long& n_walks)
 {
     using F = typename RecOf<PREC>::amp;
     constexpr int kFlushBatches = 16;   // mixed: FP32 window flushed every 16 x 64 particles
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
-    const WorkItem it = items[blockIdx.x];
     const ScreenDev S = screens[it.screen];
     const int pix = S.perm[it.tile * kLanes + lane];
     const bool active = pix >= 0;
@@ -654,13 +673,13 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         tmin = t0 - dt * 0.5;                         // particle.cpp:780-781
         tmax = t0 + dt * ((double)nb - 0.5);
     }
-    int2* __restrict__ cur_base = cursors + it.cursor_off;
     const Window<F> win{(typename Window<F>::LdsF*)(reinterpret_cast<F*>(lds)) + lane};
     // FP64 wave-uniform walk: two record chunks after the window and its trash bucket
     double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)(lds_window + 1) * 6 * kLanes);
-    unsigned long long n_steps = 0, n_walks = 0;
-    // seed the cursors of a group that does not start at bucket 0
-    if (it.bin_lo > 0) {
+    // wave-uniform walk: seed the cursors of a group that does not start at
+    // bucket 0 (the per-lane walk searches each lane's start itself)
+    const bool seeded = !LANES && it.bin_lo > 0;
+    if (seeded) {
         const int l0 = S.perm[it.tile * kLanes];
         const double Rx = S.obs[3 * l0], Ry = S.obs[3 * l0 + 1], Rz = S.obs[3 * l0 + 2], Rt0 = S.t0[l0];
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
@@ -682,7 +701,7 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
             const int npb = it.p1 - pb < kLanes ? it.p1 - pb : kLanes;
             int2 cur = make_int2(0, INT_MIN);
-            if (b0 > 0 && lane < npb) cur = cur_base[pb - it.p0 + lane];
+            if ((seeded || b0 > it.bin_lo) && lane < npb) cur = cur_base[pb - it.p0 + lane];
             unsigned long long todo = __ballot(lane < npb && cur.x < NP - 1 && cur.y < b1);
             int2 ncur = cur;
             if (todo) ++pending;
@@ -690,16 +709,22 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                 const int j = __ffsll((long long)todo) - 1;
                 todo &= todo - 1;
                 const int k = __builtin_amdgcn_readlane(cur.x, j);
-                const bool verify = __builtin_amdgcn_readlane(cur.y, j) == INT_MIN && k > 0;
+                const bool first = __builtin_amdgcn_readlane(cur.y, j) == INT_MIN;  // first walk in this item
                 const int p = pb + j;
                 int nk, nmin;
-                if constexpr (LANES)
+                if constexpr (LANES) {
+                    // per-lane offsets: read when a previous window of this
+                    // item stored them, written when a next window follows
+                    unsigned char* lc = lane_cur ? lane_cur + (size_t)(p - it.p0) * kLanes : nullptr;
                     n_steps += walk_lanes<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
-                                                tmin, tmax, nb, b0, b1, W, win, verify, nk, nmin);
-                else
+                                                tmin, tmax, nb, b0, b1, W, win, first, first ? nullptr : lc,
+                                                b1 < it.bin_hi ? lc : nullptr, nk, nmin);
+                } else {
+                    const bool verify = first && k > 0;
                     n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
                                                    inv_dt, tmin, tmax, nb, b0, b1, W, win, stage, lane, verify, nk,
                                                    nmin);
+                }
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
             }
@@ -727,20 +752,49 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
             }
         }
     }
-    if (stats && lane == 0) {
-        stats[2 * (size_t)blockIdx.x] = n_steps;
-        stats[2 * (size_t)blockIdx.x + 1] = n_walks;
+}
+
+// Persistent waves: each wave takes work items in list order from the
+// counter next_item (zero at launch) until the list is exhausted, and keeps
+// the cursors of its current item in its own slot of slot_store (slot_bytes
+// per wave: slot_particles int2 cursors, then, when lane_offsets, kLanes
+// bytes per particle).  The grid is sized to the resident waves, so the
+// cursor store scales with the machine, not with the work list.
+template <int PREC, bool LANES>
+__global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items, int n_items,
+                                                        int* __restrict__ next_item,
+                                                        const ScreenDev* __restrict__ screens,
+                                                        const typename RecOf<PREC>::type* __restrict__ recs,
+                                                        int n_points, unsigned char* __restrict__ slot_store,
+                                                        size_t slot_bytes, int slot_particles, int lane_offsets,
+                                                        unsigned long long* __restrict__ stats, int lds_window)
+{
+    unsigned char* slot = slot_store + (size_t)blockIdx.x * slot_bytes;
+    int2* cur_base = reinterpret_cast<int2*>(slot);
+    unsigned char* lane_cur = lane_offsets ? slot + (size_t)slot_particles * sizeof(int2) : nullptr;
+    for (;;) {
+        int i = 0;
+        if (threadIdx.x == 0) i = atomicAdd(next_item, 1);
+        i = __builtin_amdgcn_readfirstlane(i);
+        if (i >= n_items) break;
+        const WorkItem it = items[i];
+        unsigned long long n_steps = 0, n_walks = 0;
+        run_item<PREC, LANES>(it, screens, recs, n_points, cur_base, lane_cur, lds_window, n_steps, n_walks);
+        if (stats && threadIdx.x == 0) {
+            stats[2 * (size_t)i] = n_steps;
+            stats[2 * (size_t)i + 1] = n_walks;
+        }
     }
 }
 
-template __global__ void radiation_kernel<0, false>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
-                                                    unsigned long long*, int);
-template __global__ void radiation_kernel<1, false>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
-                                                    unsigned long long*, int);
-template __global__ void radiation_kernel<0, true>(const WorkItem*, const ScreenDev*, const RecF64*, int, int2*,
-                                                   unsigned long long*, int);
-template __global__ void radiation_kernel<1, true>(const WorkItem*, const ScreenDev*, const RecMixed*, int, int2*,
-                                                   unsigned long long*, int);
+template __global__ void radiation_kernel<0, false>(const WorkItem*, int, int*, const ScreenDev*, const RecF64*, int,
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+template __global__ void radiation_kernel<1, false>(const WorkItem*, int, int*, const ScreenDev*, const RecMixed*, int,
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+template __global__ void radiation_kernel<0, true>(const WorkItem*, int, int*, const ScreenDev*, const RecF64*, int,
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+template __global__ void radiation_kernel<1, true>(const WorkItem*, int, int*, const ScreenDev*, const RecMixed*, int,
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
 
 // ---------------------------------------------------------------------------
 // Sum the chunk partials into the trace store, chunks in order.
@@ -754,6 +808,11 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, doubl
         for (int c = 0; c < n_chunks; ++c) s += partial[(size_t)c * n + i];
         out[i] = s;
     }
+}
+
+__global__ void set_int_kernel(int* __restrict__ p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 __global__ void fill_kernel(double* __restrict__ p, size_t n, double v)

@@ -49,10 +49,13 @@ struct LaunchPlan {
     std::vector<ScreenDev> sdev;
     std::vector<int> screen_ids;
     size_t partial_doubles = 0;
-    size_t cursor_count = 0;
+    int slots = 0;              // persistent waves (grid size)
+    size_t slot_bytes = 0;      // cursor store per wave
     int max_window = 0;
     int groups = 1;
     bool lanes = true;  // per-lane walk (tg_tuning.walk == 2)
+    bool lane_cursors = false;  // per-lane cursor offsets kept between windows
+    size_t lds = 0;             // dynamic LDS per wave
 };
 
 }  // namespace
@@ -78,13 +81,17 @@ struct tg_context {
     size_t cap_items = 0;
     ScreenDev* d_sdev = nullptr;
     size_t cap_sdev = 0;
-    int2* d_cursors = nullptr;
-    size_t cap_cursors = 0;
+    unsigned char* d_slots = nullptr;  // per-wave cursor slots of the persistent radiation kernel
+    size_t cap_slots = 0;
+    int* d_next_item = nullptr;        // work-list counter of the persistent radiation kernel
+    size_t cap_next_item = 0;
     double* d_partial = nullptr;
     size_t cap_partial = 0;
     // walk statistics (diagnostic; on when TG_STATS=1 in the environment)
     bool stats_on = false;
     int default_walk = 2;
+    bool lane_cursors_on = true;  // TG_LANE_CURSORS=0 turns the per-lane offsets off
+    int max_slots = 0;            // TG_SLOTS=n caps the persistent grid at n waves (0: resident waves)
     unsigned long long* d_stats = nullptr;
     size_t cap_stats = 0;
     long long last_steps = 0, last_walks = 0, last_items = 0;
@@ -340,6 +347,12 @@ double steps_in_buckets(const tg_context* c, const ScreenHost& s, int tile, int 
     return steps;
 }
 
+const void* kernel_fn(int prec, bool lanes)
+{
+    if (prec == TG_FP64) return lanes ? (const void*)radiation_kernel<0, true> : (const void*)radiation_kernel<0, false>;
+    return lanes ? (const void*)radiation_kernel<1, true> : (const void*)radiation_kernel<1, false>;
+}
+
 // Build the work list for screens `ids`, particles [p0,p1).
 int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int p1, const tg_tuning* tun,
               LaunchPlan& plan)
@@ -404,7 +417,6 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     int groups = (int)std::max<long long>(1, std::min<long long>(64, (32 * slots + base_items - 1) / base_items));
     if (tun && tun->window_groups > 0) groups = tun->window_groups;
     plan.groups = groups;
-    long long cur = 0;
     for (int ch = 0; ch < n_chunks; ++ch) {
         int a = p0 + ch * chunk, b = std::min(p1, a + chunk);
         for (size_t si = 0; si < ids.size(); ++si) {
@@ -423,14 +435,31 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
                     w.p1 = b;
                     w.bin_lo = w_lo * W;
                     w.bin_hi = std::min(s.n_bins, w_hi * W);
-                    w.cursor_off = cur;
-                    cur += b - a;
                     plan.items.push_back(w);
                 }
             }
         }
     }
-    plan.cursor_count = (size_t)cur;
+    // Persistent grid: one wave per resident slot (LDS, registers), each
+    // with a cursor slot for one item: chunk int2 cursors and, for the
+    // per-lane walk, kLanes bytes per particle holding each lane's stopping
+    // step between windows (kept while the store stays under 16 GB; above,
+    // lanes find their start from the wave's by skipping).
+    plan.lds = lds_wave;
+    int per_cu = 0;
+    const void* kfn = kernel_fn(prec, plan.lanes);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kLanes, lds_wave) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        return fail(c, TG_ERR_HIP, "radiation kernel does not fit a compute unit");
+    }
+    plan.slots = (int)std::min<long long>((long long)plan.items.size(), (long long)std::max(c->n_cu, 1) * per_cu);
+    if (c->max_slots > 0) plan.slots = std::min(plan.slots, c->max_slots);
+    auto slot_size = [&](bool offsets) {
+        size_t b = (size_t)chunk * sizeof(int2) + (offsets ? (size_t)chunk * kLanes : 0);
+        return (b + 255) / 256 * 256;
+    };
+    plan.lane_cursors = plan.lanes && c->lane_cursors_on && (size_t)plan.slots * slot_size(true) <= ((size_t)16 << 30);
+    plan.slot_bytes = slot_size(plan.lane_cursors);
     // Dispatch the heaviest items first (longest-processing-time order), so
     // that the launch does not end on a few long waves.  The order of the
     // items changes no result: each writes its own partial-trace region.
@@ -454,7 +483,8 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 {
     int rc;
     if ((rc = ensure_cap(c, c->d_partial, c->cap_partial, plan.partial_doubles))) return rc;
-    if ((rc = ensure_cap(c, c->d_cursors, c->cap_cursors, plan.cursor_count))) return rc;
+    if ((rc = ensure_cap(c, c->d_slots, c->cap_slots, (size_t)plan.slots * plan.slot_bytes))) return rc;
+    if ((rc = ensure_cap(c, c->d_next_item, c->cap_next_item, 1))) return rc;
     if ((rc = ensure_cap(c, c->d_items, c->cap_items, plan.items.size()))) return rc;
     if ((rc = ensure_cap(c, c->d_sdev, c->cap_sdev, plan.sdev.size()))) return rc;
     if (c->stats_on && (rc = ensure_cap(c, c->d_stats, c->cap_stats, 2 * plan.items.size()))) return rc;
@@ -472,24 +502,28 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
 int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 {
     if (plan.items.empty()) return TG_OK;
-    size_t lds = (size_t)(plan.max_window + 1) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
+    set_int_kernel<<<1, 64, 0, c->stream>>>(c->d_next_item, 0);
+    TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-    const unsigned grid = (unsigned)plan.items.size();
+    const unsigned grid = (unsigned)plan.slots;
+    const int n_items = (int)plan.items.size();
     unsigned long long* st = c->stats_on ? c->d_stats : nullptr;
+    const int lo = plan.lane_cursors ? 1 : 0;
     const RecF64* r64 = (const RecF64*)c->d_rec[0];
     const RecMixed* rmx = (const RecMixed*)c->d_rec[1];
+#define TG_LAUNCH(P, L, R)                                                                                       \
+    radiation_kernel<P, L><<<grid, kLanes, plan.lds, c->stream>>>(c->d_items, n_items, c->d_next_item, c->d_sdev, \
+                                                                  R, c->n_points, c->d_slots, plan.slot_bytes,   \
+                                                                  plan.chunk, lo, st, plan.max_window)
     if (prec == TG_FP64 && plan.lanes)
-        radiation_kernel<0, true><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, r64, c->n_points,
-                                                                   c->d_cursors, st, plan.max_window);
+        TG_LAUNCH(0, true, r64);
     else if (prec == TG_FP64)
-        radiation_kernel<0, false><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, r64, c->n_points,
-                                                                    c->d_cursors, st, plan.max_window);
+        TG_LAUNCH(0, false, r64);
     else if (plan.lanes)
-        radiation_kernel<1, true><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, rmx, c->n_points,
-                                                                   c->d_cursors, st, plan.max_window);
+        TG_LAUNCH(1, true, rmx);
     else
-        radiation_kernel<1, false><<<grid, kLanes, lds, c->stream>>>(c->d_items, c->d_sdev, rmx, c->n_points,
-                                                                    c->d_cursors, st, plan.max_window);
+        TG_LAUNCH(1, false, rmx);
+#undef TG_LAUNCH
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
@@ -642,6 +676,10 @@ int tg_create(int device, tg_context** out)
     c->stats_on = st && st[0] == '1';
     const char* wk = getenv("TG_WALK");
     c->default_walk = (wk && wk[0] == '1') ? 1 : 2;
+    const char* lcv = getenv("TG_LANE_CURSORS");
+    c->lane_cursors_on = !(lcv && lcv[0] == '0');
+    const char* sl = getenv("TG_SLOTS");
+    c->max_slots = sl ? std::max(0, atoi(sl)) : 0;
     const void* kernels[] = {(const void*)radiation_kernel<0, false>, (const void*)radiation_kernel<1, false>,
                              (const void*)radiation_kernel<0, true>, (const void*)radiation_kernel<1, true>};
     for (const void* k : kernels)
@@ -666,7 +704,8 @@ int tg_destroy(tg_context* c)
     }
     (void)hipFree(c->d_items);
     (void)hipFree(c->d_sdev);
-    (void)hipFree(c->d_cursors);
+    (void)hipFree(c->d_slots);
+    (void)hipFree(c->d_next_item);
     (void)hipFree(c->d_partial);
     (void)hipFree(c->d_stats);
     (void)hipFree(c->d_flag);

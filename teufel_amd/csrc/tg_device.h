@@ -77,7 +77,6 @@ struct WorkItem {
     int chunk;
     int p0, p1;           // particle range
     int bin_lo, bin_hi;   // bucket range [bin_lo, bin_hi) of this item (a group of windows)
-    long long cursor_off; // offset of this item's cursors (p1-p0 int2)
 };
 
 // -------- IEEE-exact helpers (no contraction) ------------------------------
