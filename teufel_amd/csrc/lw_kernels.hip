@@ -258,20 +258,40 @@ struct Window {
     using LdsF = __attribute__((address_space(3))) F;
     LdsF* lane_base;  // lds + lane
     __device__ __forceinline__ LdsF* cell(int j) const { return lane_base + j * (6 * kLanes); }
-    // bucket ja += aa f1 + ba f2 and bucket jb += ab f1 + bb f2 (ja != jb)
+    // *p += v.  FP64 cells add in the LDS unit (ds_add_f64): each lane owns
+    // its column, so no two lanes meet at one address and the adds of a lane
+    // happen in program order; the VALU does not read the cell back.  FP32
+    // cells are read, added and written by the VALU (ds_add_f32 measured ten
+    // times slower than that on gfx950).
+    static __device__ __forceinline__ void add(LdsF* p, F v)
+    {
+        if constexpr (sizeof(F) == 8)
+            __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        else
+            *p += v;
+    }
+    // bucket ja += aa f1 + ba f2 and bucket jb += ab f1 + bb f2 (ja != jb,
+    // or both the trash bucket)
     __device__ __forceinline__ void add2(int ja, F aa, F ba, int jb, F ab, F bb, const F* f1, const F* f2) const
     {
         LdsF* ca = cell(ja);
         LdsF* cb = cell(jb);
-        F va[6], vb[6];
+        if constexpr (sizeof(F) == 8) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) va[c] = ca[c * kLanes];
+            for (int c = 0; c < 6; ++c) add(ca + c * kLanes, aa * f1[c] + ba * f2[c]);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) vb[c] = cb[c * kLanes];
+            for (int c = 0; c < 6; ++c) add(cb + c * kLanes, ab * f1[c] + bb * f2[c]);
+        } else {
+            F va[6], vb[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) ca[c * kLanes] = va[c] + (aa * f1[c] + ba * f2[c]);
+            for (int c = 0; c < 6; ++c) va[c] = ca[c * kLanes];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) cb[c * kLanes] = vb[c] + (ab * f1[c] + bb * f2[c]);
+            for (int c = 0; c < 6; ++c) vb[c] = cb[c * kLanes];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ca[c * kLanes] = va[c] + (aa * f1[c] + ba * f2[c]);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) cb[c * kLanes] = vb[c] + (ab * f1[c] + bb * f2[c]);
+        }
     }
 };
 
@@ -287,7 +307,7 @@ struct WindowAcc {
         auto* c0 = w.cell(j - b0);
         const F fa = (F)a, fb = (F)b;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) c0[c * kLanes] += fa * f1[c] + fb * f2[c];
+        for (int c = 0; c < 6; ++c) Window<F>::add(c0 + c * kLanes, fa * f1[c] + fb * f2[c]);
     }
 };
 
