@@ -53,7 +53,8 @@ typedef struct tg_context tg_context;
 
 /* Launch shape of the radiation kernel; zero fields mean "autotuned/default". */
 typedef struct {
-    int bins_per_window;     /* trace bins held in LDS per work item: 4, 8, 16 or 32 */
+    int bins_per_window;     /* trace bins held in LDS per wave at a time (>= 1, capped by the LDS
+                                size; 0: the autotuned or default window) */
     int particles_per_chunk; /* particles per work item (partial traces summed in order) */
     int use_graph;           /* 1: replay the launch sequence from a captured hipGraph */
     int window_groups;       /* windows of a (tile, chunk) split into this many work items */

@@ -408,13 +408,13 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     plan.partial_doubles = part_off;
     // Window groups: each (tile, chunk) is split into groups of consecutive
     // windows so that the work items fill the resident wave slots (limited
-    // by the LDS window) about 32 times over, which keeps the tail of the
+    // by the LDS window) about 16 times over, which keeps the tail of the
     // launch short; a group seeds its cursors by search instead of walking
     // from step 0.
     long long base_items = total_tiles * n_chunks;
     const size_t lds_wave = (size_t)(plan.max_window + 1) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
     long long slots = (long long)std::max(c->n_cu, 1) * (long long)std::max<size_t>(1, c->max_lds / lds_wave);
-    int groups = (int)std::max<long long>(1, std::min<long long>(64, (32 * slots + base_items - 1) / base_items));
+    int groups = (int)std::max<long long>(1, std::min<long long>(64, (16 * slots + base_items - 1) / base_items));
     if (tun && tun->window_groups > 0) groups = tun->window_groups;
     plan.groups = groups;
     for (int ch = 0; ch < n_chunks; ++ch) {
@@ -1025,9 +1025,14 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // Time each window size on a particle sample of all screens together, as
     // they are launched: every wave of a launch gets the LDS of the largest
     // window, so one window for the whole batch is what occupancy depends on.
-    const int cands[] = {3, 4, 6, 8, 12, 16};
+    // The sample is sized to about 3e10 (pixel, step) pairs per candidate
+    // (a few tenths of a second), enough for the launch to fill the machine.
+    const int cands[] = {4, 6, 8, 10, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
-    int sample = std::min(c->n_particles, std::max(64, c->n_particles / 32));
+    double pairs_per_particle = 0;
+    for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
+    const double want = 3e10 / std::max(1.0, pairs_per_particle);
+    int sample = (int)std::min<double>(c->n_particles, std::max(64.0, want));
     int max_bins = 0;
     std::vector<std::vector<double>> saved(c->screens.size());
     for (size_t id = 0; id < c->screens.size(); ++id) {
