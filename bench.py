@@ -218,7 +218,7 @@ def main():
         for i in ids:
             ctx.zero_screen(i)
         if not args.no_autotune:
-            ctx.autotune(TG_FP64)
+            log("autotune fp64: %s" % ctx.autotune(TG_FP64))
         w64, k64, t64 = time_integrate(ctx, TG_FP64, args.fp64_steps, 0)
         # one clean pass of each precision for the error (timed passes accumulate)
         for i in ids:
