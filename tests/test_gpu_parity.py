@@ -137,6 +137,22 @@ def test_mixed_energy_error_and_determinism(ctx, walk):
     assert epsW <= MIXED_ENERGY_TOL
 
 
+def test_mixed_energy_error_long_sums(ctx):
+    say("600 e- x 4000 steps in one chunk, 8x8 px, 64 coarse bins: long FP32 sums per cell; mixed vs FP64")
+    tr, obs, t0, dt, nrm = _small_bunch(n_particles=600, npx=8, n_bins=64)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 64)
+    ctx.integrate(TG_FP64)
+    S64 = ctx.screen_poynting(sid)
+    ctx.zero_screen(sid)
+    ctx.integrate(TG_MIXED, tuning={"particles_per_chunk": 600, "bins_per_window": 8})
+    Sm = ctx.screen_poynting(sid)
+    epsW, excl = energy_rel_error(W.energy_density(Sm, nrm), W.energy_density(S64, nrm))
+    print("mixed: eps_W = %.3e (excluded %.3f)" % (epsW, excl))
+    assert epsW <= MIXED_ENERGY_TOL
+
+
 @pytest.mark.parametrize("prec,walk", [(TG_FP64, 2), (TG_MIXED, 2), (TG_FP64, 1), (TG_MIXED, 1)])
 def test_persistent_waves_and_lane_offsets_bitwise(ctx, prec, walk, monkeypatch):
     say("few persistent waves (many items each), per-lane offsets off: bitwise equal; prec=%d walk=%d"

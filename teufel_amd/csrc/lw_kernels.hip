@@ -213,8 +213,8 @@ __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double 
 
 // Per precision: trajectory record, amplitude type (also the LDS window type).
 // FP64 accumulates the window in FP64 LDS; the mixed kernel accumulates an
-// FP32 LDS window that is flushed into the FP64 partial trace after every
-// batch of 64 particles, which bounds each FP32 sum to a few hundred terms.
+// FP32 LDS window that is flushed into the FP64 partial trace before its
+// cells hold much more than 2048 terms each.
 template <int PREC>
 struct RecOf;
 template <>
@@ -282,6 +282,9 @@ struct Window {
 #pragma unroll
             for (int c = 0; c < 6; ++c) add(cb + c * kLanes, ab * f1[c] + bb * f2[c]);
         } else {
+            // cell + (a f1 + b f2): one rounding at the cell's magnitude per
+            // step (folding the add into the FMAs would make it two, and
+            // the window's FP32 sums measurably worse)
             F va[6], vb[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) va[c] = ca[c * kLanes];
@@ -676,7 +679,11 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                                          int lds_window, unsigned long long& n_steps, unsigned long long& n_walks)
 {
     using F = typename RecOf<PREC>::amp;
-    constexpr int kFlushBatches = 16;   // mixed: FP32 window flushed every 16 x 64 particles
+    // Mixed: the FP32 window is flushed into the FP64 partial trace once its
+    // cells may hold about kFlushTerms summed terms (a walk of T wave-steps
+    // adds about T/W terms to each of its W cells, counted as T/W + 1), so an
+    // FP32 sum never runs much beyond 2048 terms (~2^-24 sqrt(2048) relative).
+    constexpr unsigned kFlushTerms = 2048;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const ScreenDev S = screens[it.screen];
@@ -717,57 +724,60 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         double* out = S.partial + (((size_t)it.chunk * S.n_pix + (active ? pix : 0)) * nb + b0) * 6;
         for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
         bool first_flush = true;
-        int pending = 0;
+        bool dirty = false;    // the window holds terms not yet in the partial trace
+        unsigned terms = 0;    // estimated terms per cell since the last flush
+        auto flush = [&]() {
+            if (active) {
+                for (int j = 0; j < ncell; ++j) {
+                    double v = (double)win.lane_base[j * kLanes];
+                    out[j] = first_flush ? v : out[j] + v;
+                }
+            }
+            for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
+            first_flush = false;
+            dirty = false;
+            terms = 0;
+        };
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
             const int npb = it.p1 - pb < kLanes ? it.p1 - pb : kLanes;
             int2 cur = make_int2(0, INT_MIN);
             if ((seeded || b0 > it.bin_lo) && lane < npb) cur = cur_base[pb - it.p0 + lane];
             unsigned long long todo = __ballot(lane < npb && cur.x < NP - 1 && cur.y < b1);
             int2 ncur = cur;
-            if (todo) ++pending;
             while (todo) {
                 const int j = __ffsll((long long)todo) - 1;
                 todo &= todo - 1;
                 const int k = __builtin_amdgcn_readlane(cur.x, j);
                 const bool first = __builtin_amdgcn_readlane(cur.y, j) == INT_MIN;  // first walk in this item
                 const int p = pb + j;
-                int nk, nmin;
+                int nk, nmin, ws;
                 if constexpr (LANES) {
                     // per-lane offsets: read when a previous window of this
                     // item stored them, written when a next window follows
                     unsigned char* lc = lane_cur ? lane_cur + (size_t)(p - it.p0) * kLanes : nullptr;
-                    n_steps += walk_lanes<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt,
-                                                tmin, tmax, nb, b0, b1, W, win, first, first ? nullptr : lc,
-                                                b1 < it.bin_hi ? lc : nullptr, nk, nmin);
+                    ws = walk_lanes<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin, tmax,
+                                          nb, b0, b1, W, win, first, first ? nullptr : lc,
+                                          b1 < it.bin_hi ? lc : nullptr, nk, nmin);
                 } else {
                     const bool verify = first && k > 0;
-                    n_steps += walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt,
-                                                   inv_dt, tmin, tmax, nb, b0, b1, W, win, stage, lane, verify, nk,
-                                                   nmin);
+                    ws = walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin,
+                                             tmax, nb, b0, b1, W, win, stage, lane, verify, nk, nmin);
                 }
+                n_steps += ws;
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
-            }
-            if (b1 < it.bin_hi && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
-            if constexpr (PREC == 1) {
-                // FP32 window -> FP64 partial trace every kFlushBatches batches with work
-                if (pending >= kFlushBatches) {
-                    if (active) {
-                        for (int j = 0; j < ncell; ++j) {
-                            double v = (double)win.lane_base[j * kLanes];
-                            out[j] = first_flush ? v : out[j] + v;
-                        }
-                    }
-                    for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
-                    first_flush = false;
-                    pending = 0;
+                if constexpr (PREC == 1) {
+                    dirty = true;
+                    terms += (unsigned)ws / (unsigned)(b1 - b0) + 1;
+                    if (terms >= kFlushTerms) flush();
                 }
             }
+            if (b1 < it.bin_hi && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
         }
         if (active) {
             if (PREC == 0 || first_flush) {
                 for (int j = 0; j < ncell; ++j) out[j] = (double)win.lane_base[j * kLanes];
-            } else if (pending > 0) {
+            } else if (dirty) {
                 for (int j = 0; j < ncell; ++j) out[j] += (double)win.lane_base[j * kLanes];
             }
         }

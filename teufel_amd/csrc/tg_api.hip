@@ -1025,14 +1025,22 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // Time each window size on a particle sample of all screens together, as
     // they are launched: every wave of a launch gets the LDS of the largest
     // window, so one window for the whole batch is what occupancy depends on.
-    // The sample is sized to about 3e10 (pixel, step) pairs per candidate
-    // (a few tenths of a second), enough for the launch to fill the machine.
+    // The sample is sized to about 1e11 (pixel, step) pairs per candidate
+    // (under a second), and it runs with the chunk size and window groups of
+    // the full launch, so its work items have the full launch's shape.
     const int cands[] = {4, 6, 8, 10, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     double pairs_per_particle = 0;
     for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
-    const double want = 3e10 / std::max(1.0, pairs_per_particle);
+    const double want = 1e11 / std::max(1.0, pairs_per_particle);
     int sample = (int)std::min<double>(c->n_particles, std::max(64.0, want));
+    std::vector<int> ids;
+    for (int i = 0; i < (int)c->screens.size(); ++i)
+        if (c->screens[i].n_bins >= 2 && c->screens[i].n_pix > 0) ids.push_back(i);
+    int rc;
+    if ((rc = ensure_records(c, precision))) return rc;
+    for (int id : ids)
+        if ((rc = ensure_tiles(c, c->screens[id]))) return rc;
     int max_bins = 0;
     std::vector<std::vector<double>> saved(c->screens.size());
     for (size_t id = 0; id < c->screens.size(); ++id) {
@@ -1049,8 +1057,11 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         if (w > wmax || w > max_bins) continue;
         tg_tuning t = {};
         t.bins_per_window = w;
-        int rc = integrate_impl(c, precision, &t, -1, 0, sample);
-        if (rc) return rc;
+        LaunchPlan full;
+        if ((rc = make_plan(c, precision, ids, 0, c->n_particles, &t, full))) return rc;
+        t.particles_per_chunk = full.chunk;
+        t.window_groups = full.groups;
+        if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
         if (c->kernel_ms < best_ms) {
             best_ms = c->kernel_ms;
             best_w = w;
