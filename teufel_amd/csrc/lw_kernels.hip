@@ -314,6 +314,10 @@ struct WindowAcc {
     }
 };
 
+// s_waitcnt vmcnt(0) with expcnt and lgkmcnt left open (gfx9 encoding:
+// vmcnt[3:0] in bits 3:0 and [5:4] in bits 15:14, expcnt in 6:4, lgkmcnt in 11:8).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 __device__ __forceinline__ int wave_min(int v)
 {
 #pragma unroll
@@ -621,6 +625,13 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
 #else
     Rec r1 = rp[kl + 1 < NP ? kl + 1 : NP - 1];
     while (__ballot(walking)) {
+        // Wait for the record read ahead in the previous iteration before the
+        // next one is issued: the loads stay in flight for the whole step.
+        // (Left to itself the scheduler may issue the next loads first; the
+        // wait for the old record then also waits for them, which exposes
+        // their full latency every step: 25 % slower on c2.)
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __builtin_amdgcn_sched_barrier(0);
         const Rec rn = rp[kl + 2 < NP ? kl + 2 : NP - 1];
         eval_point<PREC>(r1, Px, Py, Pz, t0, dt, inv_dt, p1);
         deposit_window<F>(win, W, walking, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
