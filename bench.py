@@ -188,6 +188,8 @@ def main():
     log("device %s, %d CUs" % (dev.strip(), n_cu))
     w, ids = build_workload(ctx, args.workload, args.particles)
     n_pairs = w.n_pairs()
+    n_inwin = ctx.count_in_window(-1)   # pairs passing the window test (particle.cpp:793)
+    log("pairs: %d, passing the window test: %d (%.4f)" % (n_pairs, n_inwin, n_inwin / n_pairs))
     prec = TG_MIXED if args.precision == "mixed" else TG_FP64
     tuning = None
     if args.window or args.graph or args.groups or args.chunk or args.walk:
@@ -263,6 +265,7 @@ def main():
         "config": {"workload": w.name, "description": w.description, "particles": w.n_particles,
                    "points_per_particle": w.n_points, "pixels": sum(s.n_pix for s in w.screens),
                    "bins": s0.n_bins, "screens": len(w.screens), "n_pairs": n_pairs,
+                   "n_inwin": n_inwin,
                    "precision": args.precision, "tuning": tuned, "graph": bool(args.graph)},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": read_traffic(PREC_NAME[prec], w.name),

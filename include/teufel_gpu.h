@@ -126,6 +126,15 @@ int tg_screen_traces_device(tg_context* ctx, int screen_id, double** traces);
  * (FieldTrace::Poynting, src/fieldtrace.cpp:192-198), out[n_pix][3] on host. */
 int tg_screen_poynting(tg_context* ctx, int screen_id, double* out);
 
+/* Number of (pixel, particle step) pairs that pass the window test of
+ * ChargedParticle::integrateFieldTrace, ts2 > tmin && ts1 < tmax
+ * (src/particle.cpp:778-793), summed over the particles and the pixels of
+ * screen screen_id (-1: all screens).  The measure of how much of the
+ * N_pairs = n_pix x n_particles x (n_points - 1) an integrate deposits; the
+ * retarded times are those of the FP64 kernel.  No counterpart in the
+ * reference, which has no such count. */
+int tg_count_in_window(tg_context* ctx, int screen_id, long long* n_inwin);
+
 /* The bucket deposit alone, driven by injected per-point retarded times and
  * fields, as the Special particle of tests/teufel.integrate_field.cpp:51-82:
  * ts[n_pseudo][n_points], fields[n_pseudo][n_points][6] (host), trace

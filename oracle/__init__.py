@@ -53,6 +53,8 @@ def lib():
         L.or_track_undulator.argtypes = [i, _dp, _dp, d, d, d, d, i, d, d, i, _dp] + [_dp] * 10 + [i]
         L.or_retarded_field.argtypes = [ctypes.c_void_p, i, i, _dp, _dp]
         L.or_retarded_time.argtypes = [ctypes.c_void_p, i, i, _dp]
+        L.or_count_in_window.argtypes = [ctypes.c_void_p, i, _dp, _dp, d, i]
+        L.or_count_in_window.restype = ctypes.c_longlong
         L.or_field_at.argtypes = [ctypes.c_void_p, i, d, _dp, _dp]
         _lib = L
     return _lib
@@ -170,6 +172,14 @@ def retarded_time(tr, p, k, obs):
     s = _or_struct(tr)
     obs = np.ascontiguousarray(obs, dtype=np.float64)
     return lib().or_retarded_time(ctypes.byref(s), p, k, _ptr(obs))
+
+
+def count_in_window(tr, obs, t0, dt, n_bins):
+    """(pixel, particle step) pairs passing the window test of particle.cpp:793."""
+    obs = np.ascontiguousarray(obs, dtype=np.float64).reshape(-1, 3)
+    t0 = np.ascontiguousarray(t0, dtype=np.float64).reshape(-1)
+    s = _or_struct(tr)
+    return int(lib().or_count_in_window(ctypes.byref(s), obs.shape[0], _ptr(obs), _ptr(t0), dt, n_bins))
 
 
 def screen_cells(origin, dX, dY, nx, ny, t0_obs, dtx, dty):

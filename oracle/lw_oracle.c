@@ -327,6 +327,27 @@ int or_integrate_screen(const or_trajectories* tr, int n_pix, const double* obs,
     return err;
 }
 
+long long or_count_in_window(const or_trajectories* tr, int n_pix, const double* obs, const double* t0,
+                             double dt, int nots)
+{
+    /* particle.cpp:778-793: the steps of every (pixel, particle) pair that
+     * pass the window test ts2 > tmin && ts1 < tmax (the silent no-ops for
+     * nots < 2 and NP < 2 count nothing) */
+    long long n = 0;
+    if (nots < 2 || tr->n_points < 2) return 0;
+    for (int i = 0; i < n_pix; i++) {
+        double tmin = t0[i] - dt * 0.5;
+        double tmax = t0[i] + dt * ((double)nots - 0.5);
+        for (int p = 0; p < tr->n_particles; p++)
+            for (int step = 0; step < tr->n_points - 1; step++) {
+                double ts1 = or_retarded_time(tr, p, step, obs + 3 * (size_t)i);
+                double ts2 = or_retarded_time(tr, p, step + 1, obs + 3 * (size_t)i);
+                if ((ts2 > tmin) && (ts1 < tmax)) n++;
+            }
+    }
+    return n;
+}
+
 void or_screen_cells(const double origin[3], const double dX[3], const double dY[3],
                      int nx, int ny, double t0_obs, double dtx, double dty,
                      double* obs, double* t0)

@@ -89,6 +89,12 @@ int or_integrate_screen(const or_trajectories* tr, int n_pix, const double* obs,
                         const double* t0, double dt, int n_bins, double* traces,
                         int n_threads);
 
+/* Number of (pixel, particle step) pairs that pass the window test of
+ * ChargedParticle::integrateFieldTrace                 src/particle.cpp:778-793
+ * (ts2 > tmin && ts1 < tmax), summed over pixels and particles. */
+long long or_count_in_window(const or_trajectories* tr, int n_pix, const double* obs, const double* t0,
+                             double dt, int n_bins);
+
 /* ScreenObserver::CellPosition / CellTimeZero     src/screen.cpp:98-110
  * Fills obs[nx*ny][3] and t0[nx*ny] in the getBuffer order of
  * src/screen.cpp:248-271 (ix outer, iy inner). */

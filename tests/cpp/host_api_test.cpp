@@ -37,6 +37,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <iostream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -377,6 +378,8 @@ int run_diffraction(const char* h5, const char* traj, const char* out)
     screen.integrate();
     const double EuJ = screen.totalEnergy() * 1.0e6;
     std::printf("radiation energy = %.17g uJ (reference test: 0.463 +- 0.020 uJ)\n", EuJ);
+    std::fflush(stdout);
+    screen.writeReport(&std::cout);
     // the dump takes the screen's own file name; write it where asked
     MeshedScreen<Bunch> copy(h5);
     std::unique_ptr<double[]> buf(screen.getBuffer());

@@ -137,6 +137,24 @@ def test_mixed_energy_error_and_determinism(ctx, walk):
     assert epsW <= MIXED_ENERGY_TOL
 
 
+def test_count_in_window_matches_oracle(ctx):
+    say("pairs passing the window test: 48 e- x 4000 steps, 16x16 px, full and narrow windows, vs oracle")
+    tr, obs, t0, dt, _ = _small_bunch()
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    s_full = ctx.add_screen(obs, t0, dt, 256)
+    s_narrow = ctx.add_screen(obs, t0 + 60 * dt, dt / 4, 128)   # a quarter of the pulse
+    s_one = ctx.add_screen(obs, t0, dt, 1)                      # one bin: a silent no-op
+    n_full = oracle.count_in_window(tr, obs, t0, dt, 256)
+    n_narrow = oracle.count_in_window(tr, obs, t0 + 60 * dt, dt / 4, 128)
+    print("oracle: full %d, narrow %d of %d pairs" % (n_full, n_narrow, obs.shape[0] * 48 * 4000))
+    assert 0 < n_narrow < n_full
+    assert ctx.count_in_window(s_full) == n_full
+    assert ctx.count_in_window(s_narrow) == n_narrow
+    assert ctx.count_in_window(s_one) == 0
+    assert ctx.count_in_window(-1) == n_full + n_narrow
+
+
 def test_mixed_energy_error_long_sums(ctx):
     say("600 e- x 4000 steps in one chunk, 8x8 px, 64 coarse bins: long FP32 sums per cell; mixed vs FP64")
     tr, obs, t0, dt, nrm = _small_bunch(n_particles=600, npx=8, n_bins=64)

@@ -168,6 +168,15 @@ def test_diffraction_screen_end_to_end(tmp_path, precision):
     assert abs(energy - 0.463) <= 0.020          # the reference test's bound
     pts, t0, dt, traces = read_dump(dump)
     assert traces.shape == (1462, 500, 6) and dt == 2e-13
+    # MeshedScreen::writeReport (src/mesh.cpp:783-821)
+    rep = r.stdout.split("Meshed Screen")[1]
+    assert "Np=1462  Nt=500" in rep
+    assert abs(float(rep.split("Energy incident on screen =")[1].split()[0]) / energy - 1) < 1e-5
+    S = np.array([oracle.trace_poynting(t, dt) for t in traces])
+    peak = int(np.argmax(np.linalg.norm(S, axis=1)))
+    assert "peak pos[%d] =" % peak in rep
+    assert abs(float(rep.split("peak energy density =")[1].split()[0]) / np.linalg.norm(S[peak]) - 1) < 1e-5
+    assert "dt  = 0.0002 ns" in rep
     pick = np.arange(0, 1462, 7)
     ref = oracle.integrate_screen(tr, pts[pick], t0[pick], dt, 500)
     if precision == "fp64":

@@ -25,6 +25,8 @@ Each oracle function is checked against something that does not restate it:
                                 after the trajectory's end; Coulomb at rest; the loop
                                 snapshot |B| = mu0 I / (2R) (tests/teufel.loop.cpp:166-172);
                                 the compiled reference, bitwise
+  or_count_in_window            closed form for a charge at rest (steps inside,
+                                across and outside the window; one-bin no-op)
   or_screen_cells               integer-division centring of src/screen.cpp:98-110
   or_trace_poynting /           constant-field closed forms (sign and units)
   or_total_energy
@@ -288,6 +290,29 @@ def test_loop_snapshot_current_loop_field():
     tr, a, w, tau = _circle()
     f = oracle.field_at(tr, 0, tau, np.array([-a, 0.0, 0.0]))
     assert abs(np.linalg.norm(f[3:]) / (MU0 * (1e-9 / tau) / (2 * a)) - 1) < 1e-3
+
+
+def test_count_in_window_closed_form():
+    # a charge at rest at the origin, a point every h: ts_k = k h + R/c with
+    # R = 1 m for both pixels.  t0 = R/c + 150.5 h and dt = 100 h put the
+    # window (particle.cpp:780-781) at [R/c + 100.5 h, R/c + 300.5 h]:
+    # steps k = 100..300 pass ts_{k+1} > tmin and ts_k < tmax.
+    h = 1e-12
+    n = 1000
+    t = np.arange(n + 1) * h
+    zero = np.zeros((n + 1, 3))
+    tr = _traj(t, zero, zero, zero, 1.0)
+    obs = np.array([[0.0, 0.0, 1.0], [0.0, 0.6, 0.8]])
+    Rc = 1.0 / C
+    dt = 100 * h
+    t0 = np.full(2, Rc + 150.5 * h)
+    assert oracle.count_in_window(tr, obs, t0, dt, 2) == 2 * 201
+    # window across the trajectory's start: [R/c - 99.5 h, R/c + 100.5 h] -> k = 0..100
+    assert oracle.count_in_window(tr, obs, t0 - 200 * h, dt, 2) == 2 * 101
+    # window wholly before the first arrival, after the last, and the one-bin no-op
+    assert oracle.count_in_window(tr, obs, t0 - 2000 * h, dt, 2) == 0
+    assert oracle.count_in_window(tr, obs, t0 + 2000 * h, dt, 2) == 0
+    assert oracle.count_in_window(tr, obs, t0, dt, 1) == 0
 
 
 # ---- whole traces against the reference's fixtures --------------------------------------

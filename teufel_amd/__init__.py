@@ -180,6 +180,13 @@ class RadiationContext:
         self._chk(self.lib.tg_screen_poynting(self.h, sid, _dptr(out)))
         return out
 
+    def count_in_window(self, sid=-1):
+        """(pixel, particle step) pairs passing the window test ts2 > tmin and
+        ts1 < tmax of src/particle.cpp:793, for screen sid (-1: all)."""
+        n = ctypes.c_longlong(0)
+        self._chk(self.lib.tg_count_in_window(self.h, sid, ctypes.byref(n)))
+        return n.value
+
     def fields_at(self, times, points, p_range=None, out=None, accumulate=False):
         """Field (E, B) at points[i] at times[i] summed over the particles
         (ChargedParticle/Bunch/Beam::RetardedField(t, x)); returns (n, 6)."""

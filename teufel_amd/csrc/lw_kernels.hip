@@ -1049,6 +1049,72 @@ __global__ void field_at_kernel(int n_obs, const double* __restrict__ obs_time, 
 }
 
 // ---------------------------------------------------------------------------
+// Pairs (pixel, particle step) that pass the window test of
+// src/particle.cpp:778-793, ts2 > tmin && ts1 < tmax, with the retarded times
+// computed exactly as the FP64 kernel (and the reference) computes them from
+// the raw trajectory arrays (t, x, y, z).  One thread per (pixel, particle),
+// pixels fastest; the retarded time is monotone
+// along a trajectory, so the first point after tmin and the last before tmax
+// are found by binary search and the passing steps lie between them.  Each
+// block adds its sum to *total.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void inwin_count_kernel(const double* __restrict__ obs,
+                                                          const double* __restrict__ t0v, int n_pix, double dt,
+                                                          double inv_dt, int n_bins,
+                                                          const double* __restrict__ rt,
+                                                          const double* __restrict__ rx,
+                                                          const double* __restrict__ ry,
+                                                          const double* __restrict__ rz, int n_particles, int NP,
+                                                          unsigned long long* __restrict__ total)
+{
+    __shared__ unsigned long long part[256 / kLanes];
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt = 0;
+    if (gid < (size_t)n_pix * n_particles) {
+        const int i = (int)(gid % n_pix), p = (int)(gid / n_pix);
+        const double Px = obs[3 * i], Py = obs[3 * i + 1], Pz = obs[3 * i + 2], t0 = t0v[i];
+        double tmin, tmax;
+        {
+#pragma clang fp contract(off)
+            tmin = t0 - dt * 0.5;                      // particle.cpp:780-781
+            tmax = t0 + dt * ((double)n_bins - 0.5);
+        }
+        const size_t base = (size_t)p * NP;
+        auto ts_of = [&](int k) {
+            double R, ts;
+            int idx;
+            const size_t g = base + k;
+            timing_exact(rt[g], Px - rx[g], Py - ry[g], Pz - rz[g], t0, dt, inv_dt, R, ts, idx);
+            return ts;
+        };
+        int lo = 0, hi = NP;  // a = first point with ts > tmin (NP: none)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ts_of(mid) > tmin) hi = mid; else lo = mid + 1;
+        }
+        const int a = lo;
+        lo = 0;
+        hi = NP;              // first point with ts >= tmax; b = the one before it
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ts_of(mid) >= tmax) hi = mid; else lo = mid + 1;
+        }
+        const int b = lo - 1;
+        // step k passes when point k+1 is after tmin (k >= a-1) and point k before tmax (k <= b)
+        const int k0 = a - 1 > 0 ? a - 1 : 0, k1 = b < NP - 2 ? b : NP - 2;
+        cnt = k1 >= k0 ? (unsigned long long)(k1 - k0 + 1) : 0ull;
+    }
+    for (int off = kLanes / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if ((threadIdx.x & (kLanes - 1)) == 0) part[threadIdx.x / kLanes] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < (int)(blockDim.x / kLanes); ++w) s += part[w];
+        atomicAdd(total, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Per-pixel Poynting integral S = dt * sum E x B / mu0 (src/fieldtrace.cpp:192-198).
 // ---------------------------------------------------------------------------
 __global__ void poynting_kernel(const double* __restrict__ traces, int n_pix, int n_bins, double dt,

@@ -918,6 +918,36 @@ int tg_screen_poynting(tg_context* c, int id, double* out)
     return TG_OK;
 }
 
+int tg_count_in_window(tg_context* c, int screen_id, long long* n_inwin)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (!n_inwin) return fail(c, TG_ERR_ARG, "null output");
+    if (screen_id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    *n_inwin = 0;
+    if (c->n_points < 2 || c->n_particles == 0) return TG_OK;  // particle.cpp:783
+    if (!c->traj_set) return fail(c, TG_ERR_STATE, "tg_count_in_window: no trajectories set");
+    TG_HIP(c, hipSetDevice(c->device));
+    unsigned long long* d = nullptr;
+    TG_HIP(c, hipMalloc((void**)&d, sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long), c->stream);
+    for (int id = 0; id < (int)c->screens.size() && e == hipSuccess; ++id) {
+        const ScreenHost& s = c->screens[id];
+        if ((screen_id >= 0 && id != screen_id) || s.n_bins < 2 || s.n_pix == 0) continue;  // particle.cpp:778
+        const size_t n = (size_t)s.n_pix * c->n_particles;
+        inwin_count_kernel<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+            s.d_obs, s.d_t0, s.n_pix, s.dt, 1.0 / s.dt, s.n_bins, raw(c, 0), raw(c, 1), raw(c, 2), raw(c, 3),
+            c->n_particles, c->n_points, d);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_count_in_window: ") + hipGetErrorString(e));
+    *n_inwin = (long long)h;
+    return TG_OK;
+}
+
 int tg_deposit_only(tg_context* c, int n_pseudo, int n_points, const double* ts, const double* fields, double t0,
                     double dt, int n_bins, double* trace)
 {

@@ -1,6 +1,8 @@
 // screens.cpp - MeshedScreen, ScreenObserver and PointObserver for Beam and Bunch.
 #include "screens.h"
 
+#include <ostream>
+
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -188,6 +190,39 @@ double MeshedScreen<SourceT>::totalEnergy() const
     double total = 0.0;
     for (int ip = 0; ip < Np; ip++) total -= area[ip] * dot(A[ip]->Poynting(), normal[ip]);
     return total;
+}
+
+template <typename SourceT>
+void MeshedScreen<SourceT>::writeReport(std::ostream* st) const
+{
+    std::ostream& o = *st;
+    const std::streamsize keep = o.precision();
+    o << "\n   Meshed Screen\n----------------------\n";
+    o << "Np=" << Np << "  Nt=" << Nt << "\n";
+    o.precision(6);
+    o << "total area = " << get_total_area() * 1.0e4 << " cm²\n";
+    o << "Energy incident on screen = " << totalEnergy() * 1.0e6 << " µJ\n";
+    int peak_ip = -1;
+    double peak = 0.0;
+    Vector S_peak;
+    for (int ip = 0; ip < Np; ip++) {
+        const Vector S = A[ip]->Poynting();
+        if (S.norm() > peak) {
+            peak = S.norm();
+            peak_ip = ip;
+            S_peak = S;
+        }
+    }
+    if (peak_ip >= 0) {
+        const Vector& p = points[peak_ip];
+        o << "peak pos[" << peak_ip << "] =   (" << p.x << ", " << p.y << ", " << p.z << ") m   \n";
+        o << "peak energy density = " << peak << " J/m²\n";
+        o << "Poynting vector S = (" << S_peak.x << ", " << S_peak.y << ", " << S_peak.z << ") J/m²\n";
+        o << "t0  = " << A[peak_ip]->get_t0() * 1e9 << " ns  \n";
+        o << "dt  = " << dt * 1.0e9 << " ns\n";
+    }
+    o << std::endl;
+    o.precision(keep);
 }
 
 template <typename SourceT>

@@ -8,6 +8,7 @@
 // writeDump); the reference's HDF5/SDDS writers are not part of this library.
 #pragma once
 
+#include <ostream>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,11 @@ public:
     Vector Poynting(int ip) const { return A[ip]->Poynting(); }
     // -sum_p area_p S_p . normal_p [J] (src/mesh.cpp:744-754)
     double totalEnergy() const;
+    // Summary in the reference's layout (src/mesh.cpp:783-821): Np and Nt,
+    // total area [cm²], incident energy [µJ], and the cell of the largest
+    // |Poynting| (first one on ties) with its position, energy density, S
+    // vector, t0 and dt [ns]
+    void writeReport(std::ostream* st) const;
 
     void integrate() override;
     std::size_t getBufferSize() override { return (std::size_t)Np * Nt * 6; }
