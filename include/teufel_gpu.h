@@ -126,6 +126,18 @@ int tg_screen_traces_device(tg_context* ctx, int screen_id, double** traces);
  * (FieldTrace::Poynting, src/fieldtrace.cpp:192-198), out[n_pix][3] on host. */
 int tg_screen_poynting(tg_context* ctx, int screen_id, double* out);
 
+/* Spectral power density of the two in-plane polarisations of every pixel
+ * [J/(m² Hz)], computed on device (rocFFT): with EH = E.ex, EV = E.ey,
+ * BH = B.ex, BV = B.ey, dens_h[p][i] = |F(EH) conj(F(BV))|_i / mu0 * 2 dt^2
+ * and dens_v likewise from EV and BH, for i < n_bins/2; freq[i] =
+ * i (1/dt) / (n_bins - 1).  Replaces spectralDensityH / spectralDensityV of
+ * the reference's post-processing (scripts/teufel.py:198-226, the same
+ * frequency axis, np.linspace(0, 1/dt, n_bins)[:n_bins//2]), with mu0 of
+ * src/global.h:37-44.  ex, ey: unit vectors of the screen plane (3 doubles
+ * each); freq may be null; dens_h, dens_v: [n_pix][n_bins/2] on host. */
+int tg_screen_spectra(tg_context* ctx, int screen_id, const double* ex, const double* ey, double* freq,
+                      double* dens_h, double* dens_v);
+
 /* Number of (pixel, particle step) pairs that pass the window test of
  * ChargedParticle::integrateFieldTrace, ts2 > tmin && ts1 < tmax
  * (src/particle.cpp:778-793), summed over the particles and the pixels of

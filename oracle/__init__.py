@@ -190,6 +190,28 @@ def screen_cells(origin, dX, dY, nx, ny, t0_obs, dtx, dty):
     return obs, t0
 
 
+def spectral_density(traces, dt, ex, ey):
+    """spectralDensityH / spectralDensityV of scripts/teufel.py:198-226 for
+    every trace (in-plane fields as getInPlaneFields, :167-176), with the
+    reference's C++ mu0 = 4 pi 1e-7 (src/global.h:37-44); returns
+    (f, dens_h[n_pix, nf//2], dens_v[n_pix, nf//2])."""
+    traces = np.asarray(traces, dtype=np.float64)
+    nf = traces.shape[1]
+    mu0 = 4.0 * np.pi * 1e-7
+    fmax = 1.0 / dt
+    f = np.linspace(0.0, fmax, nf)[:nf // 2]
+    df = 1.0 / dt / nf
+    E, B = traces[..., :3], traces[..., 3:]
+    EH, EV = E @ np.asarray(ex, dtype=np.float64), E @ np.asarray(ey, dtype=np.float64)
+    BH, BV = B @ np.asarray(ex, dtype=np.float64), B @ np.asarray(ey, dtype=np.float64)
+
+    def dens(e, b):
+        se = np.fft.fft(e, axis=-1)[:, :nf // 2]
+        sb = np.fft.fft(b, axis=-1)[:, :nf // 2]
+        return np.abs(se * np.conj(sb) / mu0) * 2 * dt / (df * nf)
+    return f, dens(EH, BV), dens(EV, BH)
+
+
 def trace_poynting(trace, dt):
     trace = np.ascontiguousarray(trace, dtype=np.float64)
     S = np.zeros(3)

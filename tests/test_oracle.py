@@ -27,6 +27,9 @@ Each oracle function is checked against something that does not restate it:
                                 the compiled reference, bitwise
   or_count_in_window            closed form for a charge at rest (steps inside,
                                 across and outside the window; one-bin no-op)
+  spectral_density (numpy)      closed forms: a plane wave sampled on the bins
+                                (one line of known height per polarisation, zero
+                                elsewhere) and the one-sided Parseval sum
   or_screen_cells               integer-division centring of src/screen.cpp:98-110
   or_trace_poynting /           constant-field closed forms (sign and units)
   or_total_energy
@@ -313,6 +316,38 @@ def test_count_in_window_closed_form():
     assert oracle.count_in_window(tr, obs, t0 - 2000 * h, dt, 2) == 0
     assert oracle.count_in_window(tr, obs, t0 + 2000 * h, dt, 2) == 0
     assert oracle.count_in_window(tr, obs, t0, dt, 1) == 0
+
+
+def test_spectral_density_closed_forms():
+    # a plane wave along z on bin k0, E along ex with amplitude a (EH) and
+    # along ey with amplitude b (EV), B = z x E / c: F(EH)_k0 = a nf/2,
+    # F(BV)_k0 = a nf/(2c), so dens_h = (a nf/2)^2 / (c mu0) * 2 dt^2 there
+    # and 0 elsewhere; likewise dens_v with b^2 (F(BH) = -b nf/(2c))
+    nf, dt, k0, a, b = 256, 1e-13, 17, 3.0, 0.5
+    t = np.arange(nf)
+    ex, ey = np.array([1.0, 0.0, 0.0]), np.array([0.0, 1.0, 0.0])
+    wave = np.cos(2 * np.pi * k0 * t / nf)
+    E = np.stack([a * wave, b * wave, 0 * wave], -1)
+    B = np.cross(np.array([0.0, 0.0, 1.0]), E) / C
+    tr = np.concatenate([E, B], -1)[None]
+    f, dh, dv = oracle.spectral_density(tr, dt, ex, ey)
+    assert f.shape == (nf // 2,) and abs(f[1] - 1 / dt / (nf - 1)) < 1e-3
+    peak_h = (a * nf / 2) ** 2 / (C * MU0) * 2 * dt * dt
+    peak_v = (b * nf / 2) ** 2 / (C * MU0) * 2 * dt * dt
+    assert abs(dh[0, k0] / peak_h - 1) < 1e-12 and abs(dv[0, k0] / peak_v - 1) < 1e-12
+    assert np.delete(dh[0], k0).max() < 1e-12 * peak_h and np.delete(dv[0], k0).max() < 1e-12 * peak_v
+    # one-sided Parseval for a random far-field trace (B = z x E / c): the
+    # densities sum, times df, to the energy of that polarisation
+    rng = np.random.default_rng(5)
+    E = np.concatenate([rng.normal(size=(nf, 2)), np.zeros((nf, 1))], -1)
+    E -= E.mean(axis=0)                                     # no DC line
+    B = np.cross(np.array([0.0, 0.0, 1.0]), E) / C
+    tr = np.concatenate([E, B], -1)[None]
+    f, dh, dv = oracle.spectral_density(tr, dt, ex, ey)
+    df = 1 / dt / nf
+    energy_h = dt * np.sum(E[:, 0] * B[:, 1]) / MU0
+    nyq = abs(np.fft.fft(E[:, 0])[nf // 2]) ** 2 / (C * MU0) * dt * dt   # the Nyquist line, not in the sum
+    assert abs((dh.sum() * df + nyq * df) / energy_h - 1) < 1e-12
 
 
 # ---- whole traces against the reference's fixtures --------------------------------------

@@ -180,6 +180,21 @@ class RadiationContext:
         self._chk(self.lib.tg_screen_poynting(self.h, sid, _dptr(out)))
         return out
 
+    def screen_spectra(self, sid, ex, ey):
+        """Spectral power density [J/(m² Hz)] of the horizontal (E.ex, B.ey)
+        and vertical (E.ey, B.ex) polarisations of every pixel, as
+        spectralDensityH/V of scripts/teufel.py:198-226; returns
+        (f[n_bins//2], dens_h[n_pix, n_bins//2], dens_v[n_pix, n_bins//2])."""
+        n_pix, n_bins = self.screens[sid]
+        nh = n_bins // 2
+        ex = np.ascontiguousarray(ex, dtype=np.float64)
+        ey = np.ascontiguousarray(ey, dtype=np.float64)
+        f = np.zeros(max(nh, 1))
+        dh = np.zeros((n_pix, max(nh, 1)))
+        dv = np.zeros((n_pix, max(nh, 1)))
+        self._chk(self.lib.tg_screen_spectra(self.h, sid, _dptr(ex), _dptr(ey), _dptr(f), _dptr(dh), _dptr(dv)))
+        return f[:nh], dh[:, :nh], dv[:, :nh]
+
     def count_in_window(self, sid=-1):
         """(pixel, particle step) pairs passing the window test ts2 > tmin and
         ts1 < tmax of src/particle.cpp:793, for screen sid (-1: all)."""

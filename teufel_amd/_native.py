@@ -41,6 +41,7 @@ SIGNATURES = [
     ("tg_screen_traces_device", _i, [_vp, _i, ctypes.POINTER(_vp)]),
     ("tg_screen_poynting", _i, [_vp, _i, _dp]),
     ("tg_count_in_window", _i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong)]),
+    ("tg_screen_spectra", _i, [_vp, _i, _dp, _dp, _dp, _dp, _dp]),
     ("tg_deposit_only", _i, [_vp, _i, _i, _dp, _dp, _d, _d, _i, _dp]),
     ("tg_fields_at", _i, [_vp, _i, _dp, _dp, _i, _i, _dp, _i]),
     ("tg_autotune", _i, [_vp, _i, ctypes.POINTER(TgTuning)]),

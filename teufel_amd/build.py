@@ -23,8 +23,11 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-command-line-argument"]
 
 
+LIBS = ["-lrocfft"]   # per-pixel spectra (tg_screen_spectra)
+
+
 def command(out=LIB_PATH):
-    return [HIPCC] + FLAGS + ["-o", out] + SOURCES
+    return [HIPCC] + FLAGS + ["-o", out] + SOURCES + LIBS
 
 
 def build(verbose=False):

@@ -1115,6 +1115,43 @@ __global__ __launch_bounds__(256) void inwin_count_kernel(const double* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Spectra of the in-plane fields (scripts/teufel.py:167-226): per pixel and
+// sample, EH = E.ex, EV = E.ey, BH = B.ex, BV = B.ey; sig holds four signals
+// of n_bins samples per pixel, [EH | BV | EV | BH][n_pix][n_bins].
+// ---------------------------------------------------------------------------
+__global__ void inplane_kernel(const double* __restrict__ traces, size_t n, double ex0, double ex1, double ex2,
+                               double ey0, double ey1, double ey2, double* __restrict__ sig)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const double* f = traces + 6 * j;
+        sig[j] = f[0] * ex0 + f[1] * ex1 + f[2] * ex2;              // EH
+        sig[n + j] = f[3] * ey0 + f[4] * ey1 + f[5] * ey2;          // BV
+        sig[2 * n + j] = f[0] * ey0 + f[1] * ey1 + f[2] * ey2;      // EV
+        sig[3 * n + j] = f[3] * ex0 + f[4] * ex1 + f[5] * ex2;      // BH
+    }
+}
+
+// |F(EH) conj(F(BV))| / mu0 * 2 dt / (df nf) with df = 1/(nf dt), i.e. * 2 dt^2,
+// for the first n_bins/2 frequencies (scripts/teufel.py:198-211); the same
+// for EV, BH (:213-226).  spec: the four signals' one-sided transforms,
+// n_bins/2 + 1 complex values each.
+__global__ void spectral_density_kernel(const double2* __restrict__ spec, int n_pix, int n_bins, double scale,
+                                        double* __restrict__ dens_h, double* __restrict__ dens_v)
+{
+    const int nh = n_bins / 2, nc = n_bins / 2 + 1;
+    const size_t n = (size_t)n_pix * nh, sig = (size_t)n_pix * nc;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const size_t p = j / nh, i = j % nh, k = p * nc + i;
+        const double2 eh = spec[k], bv = spec[sig + k], ev = spec[2 * sig + k], bh = spec[3 * sig + k];
+        // |a conj(b)| = |a| |b|
+        dens_h[j] = sqrt(eh.x * eh.x + eh.y * eh.y) * sqrt(bv.x * bv.x + bv.y * bv.y) * scale;
+        dens_v[j] = sqrt(ev.x * ev.x + ev.y * ev.y) * sqrt(bh.x * bh.x + bh.y * bh.y) * scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Per-pixel Poynting integral S = dt * sum E x B / mu0 (src/fieldtrace.cpp:192-198).
 // ---------------------------------------------------------------------------
 __global__ void poynting_kernel(const double* __restrict__ traces, int n_pix, int n_bins, double dt,

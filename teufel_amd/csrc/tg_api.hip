@@ -10,11 +10,14 @@
 
 #include "../../include/teufel_gpu.h"
 
+#include <rocfft/rocfft.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -945,6 +948,77 @@ int tg_count_in_window(tg_context* c, int screen_id, long long* n_inwin)
     (void)hipFree(d);
     if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_count_in_window: ") + hipGetErrorString(e));
     *n_inwin = (long long)h;
+    return TG_OK;
+}
+
+int tg_screen_spectra(tg_context* c, int id, const double* ex, const double* ey, double* freq, double* dens_h,
+                      double* dens_v)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (id < 0 || id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
+    if (!ex || !ey) return fail(c, TG_ERR_ARG, "null screen axis");
+    const ScreenHost& s = c->screens[id];
+    const int nb = s.n_bins, nh = s.n_bins / 2;
+    if (freq)  // np.linspace(0, 1/dt, nb)[:nb//2] (scripts/teufel.py:204-206)
+        for (int i = 0; i < nh; ++i) freq[i] = i * ((1.0 / s.dt) / (nb - 1));
+    if (s.n_pix == 0 || nh == 0) return TG_OK;
+    if (!dens_h || !dens_v) return fail(c, TG_ERR_ARG, "null output buffer");
+    TG_HIP(c, hipSetDevice(c->device));
+    static std::once_flag fft_once;
+    std::call_once(fft_once, [] { rocfft_setup(); });
+    const size_t n = (size_t)s.n_pix * nb, nc = (size_t)nb / 2 + 1, nout = (size_t)s.n_pix * nh;
+    double *sig = nullptr, *dh = nullptr, *dv = nullptr;
+    double2* spec = nullptr;
+    void* work = nullptr;
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    std::string what;
+    hipError_t e = hipMalloc((void**)&sig, 4 * n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&spec, 4 * (size_t)s.n_pix * nc * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc((void**)&dh, nout * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&dv, nout * sizeof(double));
+    if (e == hipSuccess) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+        inplane_kernel<<<blocks, 256, 0, c->stream>>>(s.d_traces, n, ex[0], ex[1], ex[2], ey[0], ey[1], ey[2], sig);
+        e = hipGetLastError();
+    }
+    // one-sided transforms of the 4 n_pix real signals (batched, out of place)
+    const size_t len = (size_t)nb;
+    if (e == hipSuccess) {
+        size_t wsz = 0;
+        if (rocfft_plan_create(&plan, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                               rocfft_precision_double, 1, &len, 4 * (size_t)s.n_pix, nullptr) != rocfft_status_success ||
+            rocfft_plan_get_work_buffer_size(plan, &wsz) != rocfft_status_success ||
+            rocfft_execution_info_create(&info) != rocfft_status_success ||
+            rocfft_execution_info_set_stream(info, c->stream) != rocfft_status_success)
+            what = "rocFFT plan";
+        else if (wsz && (e = hipMalloc(&work, wsz)) == hipSuccess &&
+                 rocfft_execution_info_set_work_buffer(info, work, wsz) != rocfft_status_success)
+            what = "rocFFT work buffer";
+        if (what.empty() && e == hipSuccess) {
+            void* in[1] = {sig};
+            void* out[1] = {spec};
+            if (rocfft_execute(plan, in, out, info) != rocfft_status_success) what = "rocFFT execute";
+        }
+    }
+    if (e == hipSuccess && what.empty()) {
+        const double scale = 2.0 * s.dt * s.dt / kMuNull;
+        const unsigned blocks = (unsigned)std::min<size_t>((nout + 255) / 256, 16384);
+        spectral_density_kernel<<<blocks, 256, 0, c->stream>>>(spec, s.n_pix, nb, scale, dh, dv);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(dens_h, dh, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dens_v, dv, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (info) rocfft_execution_info_destroy(info);
+    if (plan) rocfft_plan_destroy(plan);
+    (void)hipFree(work);
+    (void)hipFree(sig);
+    (void)hipFree(spec);
+    (void)hipFree(dh);
+    (void)hipFree(dv);
+    if (!what.empty()) return fail(c, TG_ERR_HIP, "tg_screen_spectra: " + what + " failed");
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_screen_spectra: ") + hipGetErrorString(e));
     return TG_OK;
 }
 
