@@ -90,6 +90,18 @@ def time_integrate(ctx, prec, steps, warmup, tuning=None):
     return wall, float(np.mean([k for k, _ in kms])) / 1e3, float(np.mean([t for _, t in kms])) / 1e3
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(ctx, w, target_s=15.0, max_particles=64):
     """Reference integrateFieldTrace (compiled from the reference sources) on the
     host cores; falls back to the oracle restatement if the reference build is
@@ -125,6 +137,7 @@ def cpu_baseline(ctx, w, target_s=15.0, max_particles=64):
     dt1 = run(pix, npc2)
     pairs = len(pix) * npc2 * (w.n_points - 1)
     return {"value": pairs / dt1, "unit": "LW field evals/s", "cores": n_threads, "kind": kind,
+            "cpu": cpu_model(),
             "sample": "%d particles x %d steps x %d of %d pixels (every %d-th), %d bins, %.1f s"
                       % (npc2, w.n_points - 1, len(pix), s.n_pix, max(1, s.n_pix // max(1, len(pix))),
                          s.n_bins, dt1)}
