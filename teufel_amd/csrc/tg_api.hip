@@ -1132,7 +1132,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // The sample is sized to about 1e11 (pixel, step) pairs per candidate
     // (under a second), and it runs with the chunk size and window groups of
     // the full launch, so its work items have the full launch's shape.
-    const int cands[] = {4, 6, 8, 10, 12, 16};
+    const int cands[] = {4, 5, 6, 8, 10, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     double pairs_per_particle = 0;
     for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
