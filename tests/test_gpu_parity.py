@@ -279,6 +279,33 @@ def test_gpu_pusher_matches_oracle(ctx):
         assert np.abs(d[n] - getattr(tr, n)).max() <= 1e-10 * amax
 
 
+@pytest.mark.parametrize("prec", [TG_FP64, TG_MIXED])
+def test_extreme_shapes_vs_oracle(ctx, prec):
+    say("extreme shapes vs oracle: 2-point trajectories, 2 bins, 1 pixel, 65 pixels, 20000 bins; prec=%d" % prec)
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=5, n_steps=600, npx=16, n_bins=64)
+    cases = [
+        (tr.subset(0, 5), obs[:1], t0[:1] + 15.5 * dt, 32 * dt, 2),  # 1 pixel, 2 bins over the pulse
+        (tr.subset(0, 5), obs[:65], t0[:65], dt, 64),              # one full tile and one pixel
+        (tr.subset(0, 2), obs[:70], t0[:70], dt / 300.0, 20000),   # very long traces, fine bins
+    ]
+    short = oracle.Trajectories(**{n: np.ascontiguousarray(getattr(tr, n)[:, 300:302])
+                                   for n in oracle.Trajectories.NAMES}, charge=tr.charge)
+    ts = [oracle.retarded_time(short, p, k, obs[0]) for p in range(5) for k in range(2)]
+    cases.append((short, obs[:64], np.full(64, min(ts)), (max(ts) - min(ts)) / 3.0, 4))  # one step per particle
+    for trc, o, t0c, dtc, nb in cases:
+        ref = oracle.integrate_screen(trc, o, t0c, dtc, nb)
+        ctx.remove_screens()
+        ctx.set_trajectories(trc)
+        sid = ctx.add_screen(o, t0c, dtc, nb)
+        ctx.integrate(prec)
+        out = ctx.read_traces(sid)
+        assert np.abs(ref).max() > 0, (o.shape, nb)
+        if prec == TG_FP64:
+            assert trace_rel_error(out, ref) <= FP64_TOL, (o.shape, nb)
+        else:
+            assert trace_rel_error(out, ref) <= 1e-4, (o.shape, nb)
+
+
 def test_batched_screens_equal_single(ctx):
     say("3 uneven screens batched vs one at a time, FP64, bitwise")
     tr, obs, t0, dt, _ = _small_bunch(n_particles=24, n_steps=2000)
