@@ -351,6 +351,45 @@ def test_graph_replay_matches_eager(ctx):
         assert np.array_equal(out, eager), rep
 
 
+@pytest.mark.parametrize("prec", [TG_FP64, TG_MIXED])
+def test_autotune_keeps_traces_and_tuned_launch_matches_oracle(ctx, prec):
+    say("autotune (window, groups, chunk) on 2 screens, then tuned launch vs oracle, prec=%d" % prec)
+    tr, obs, t0, dt, nrm = _small_bunch(n_particles=40, n_steps=2000)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    s0 = ctx.add_screen(obs, t0, dt, 256)
+    o2, n2, _ = W.grid_screen(6, 0.05, 5.0, tilt_deg=15.0)
+    u = W.Undulator()
+    t_c, x_c = W._centre(u, 60.0)
+    t02, dt2 = W.set_windows(o2, 130, u, 60.0, t_c, x_c, 0.3e-3)
+    s1 = ctx.add_screen(o2, t02, dt2, 130)
+    ctx.integrate(TG_FP64, tuning={"bins_per_window": 8})
+    before = [ctx.read_traces(s0), ctx.read_traces(s1)]
+    tuned = ctx.autotune(prec)
+    print("tuned:", tuned)
+    assert 1 <= tuned["bins_per_window"] <= 256
+    assert tuned["window_groups"] >= 0 and tuned["particles_per_chunk"] >= 0
+    # the timing trials restore every screen's traces
+    assert np.array_equal(ctx.read_traces(s0), before[0])
+    assert np.array_equal(ctx.read_traces(s1), before[1])
+    ctx.zero_screen(s0)
+    ctx.zero_screen(s1)
+    ctx.integrate(prec)   # tuning=None: the autotuned shape
+    def energy(tr_, d_t, nrm_):   # dt sum E x B / mu0 through each pixel
+        S = d_t * np.cross(tr_[:, :, 0:3], tr_[:, :, 3:6]).sum(axis=1) / (4e-7 * math.pi)
+        return W.energy_density(S, nrm_)
+
+    for sid, (o, t_0, d_t, nb, nrm_) in ((s0, (obs, t0, dt, 256, nrm)), (s1, (o2, t02, dt2, 130, n2))):
+        ref = oracle.integrate_screen(tr, o, t_0, d_t, nb)
+        out = ctx.read_traces(sid)
+        if prec == TG_FP64:
+            assert trace_rel_error(out, ref) <= FP64_TOL, sid
+        else:
+            epsW, _ = energy_rel_error(energy(out, d_t, nrm_), energy(ref, d_t, nrm_))
+            print("screen %d: eps_W = %.3e" % (sid, epsW))
+            assert epsW <= MIXED_ENERGY_TOL, sid
+
+
 def test_accumulate_and_noops(ctx):
     say("accumulate semantics, n_bins<2 and NP<2 no-ops")
     tr, obs, t0, dt, _ = _small_bunch(n_particles=8, n_steps=1000)

@@ -6,19 +6,23 @@
 // (src/bunch.cpp:739-745, src/beam.cpp:443-450).
 //
 // Decomposition.  One wavefront = 64 pixels (one per lane, a "tile") x one
-// chunk of particles.  The tile's traces are produced window by window: a
-// window of W bins x 6 components x 64 lanes lives in LDS (FP64).  For every
-// particle of the chunk the wave walks the trajectory steps whose retarded-time
-// interval touches the window, evaluating each trajectory point once (the
-// reference evaluates each point twice) and depositing with the reference's
-// 3-case bucket rule.  A lane only ever writes its own pixel's LDS cells, in
-// particle order and step order, so the deposit needs no atomics and is
-// bitwise reproducible.  Per (work item, particle) a cursor remembers the
-// first step that reaches the next window, so every step is walked about
-// once per work item.  Trajectory points are uniform across the wave and are
-// read with scalar loads one step ahead of use.  Particle chunks write partial
-// traces that a second kernel sums in fixed chunk order into the screen's
-// trace store.
+// chunk of particles x a group of bucket windows (a work item); persistent
+// waves take items from a work list.  The tile's traces are produced window
+// by window: a window of W bins x 6 components x 64 lanes lives in LDS (FP64
+// cells for the FP64 kernel, FP32 cells flushed into FP64 for the mixed one).
+// For every particle of the chunk each lane walks its own pixel's trajectory
+// steps whose retarded-time interval touches the window, evaluating each
+// trajectory point once (the reference evaluates each point twice) and
+// depositing with the reference's 3-case bucket rule.  A lane only ever
+// writes its own pixel's LDS cells, in particle order and step order, so the
+// deposit needs no atomics and is bitwise reproducible.  Per particle a
+// cursor (the wave's earliest stopping step, plus one-byte lane offsets from
+// it) carries each lane's position into the next window, so every step is
+// walked about once per work item.  Records are read per lane one step ahead
+// of use.  Particle chunks write partial traces that a second kernel sums in
+// fixed chunk order into the screen's trace store.  The wave-uniform walk
+// (all lanes on one step, records read with scalar loads or staged through
+// LDS) is kept as an alternative.
 #include "tg_device.h"
 #include "tg_deposit.h"
 

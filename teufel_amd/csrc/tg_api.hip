@@ -182,6 +182,8 @@ int alloc_traj(tg_context* c, int n_particles, int n_points)
     c->n_particles = n_particles;
     c->n_points = n_points;
     c->traj_set = true;
+    // the tuned chunk and group counts were timed on the previous set
+    for (tg_tuning& t : c->tuned) t.particles_per_chunk = t.window_groups = 0;
     return TG_OK;
 }
 
@@ -1157,6 +1159,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     }
     float best_ms = 1e30f;
     int best_w = default_window(c, max_bins);
+    LaunchPlan best_full;
     for (int w : cands) {
         if (w > wmax || w > max_bins) continue;
         tg_tuning t = {};
@@ -1169,6 +1172,41 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         if (c->kernel_ms < best_ms) {
             best_ms = c->kernel_ms;
             best_w = w;
+            best_full = full;
+        }
+    }
+    // Second stage, at the chosen window: the number of window groups (item
+    // granularity against the search each group costs) and the particle
+    // chunk (items per tile against partial traces to reduce), each halved
+    // and doubled around the default.  A shape replaces the default only
+    // when it is more than 2 % faster, so timing noise does not move it.
+    if (best_full.chunk > 0) {
+        const int g0 = best_full.groups, ch0 = best_full.chunk;
+        int best_g = g0, best_ch = ch0;
+        const float base_ms = best_ms;
+        float shape_ms = base_ms;
+        size_t per_chunk = 0;
+        for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
+        const int shapes[4][2] = {{g0 / 2, ch0}, {g0 * 2, ch0}, {g0, ch0 * 2}, {g0, ch0 / 2}};
+        for (const auto& sh : shapes) {
+            const int g = sh[0], ch = sh[1];
+            if (g < 1 || g > 64 || ch < 1 || ch > c->n_particles) continue;
+            const size_t n_ch = ((size_t)c->n_particles + ch - 1) / ch;
+            if (n_ch * per_chunk * sizeof(double) > ((size_t)16 << 30)) continue;
+            tg_tuning t = {};
+            t.bins_per_window = best_w;
+            t.particles_per_chunk = ch;
+            t.window_groups = g;
+            if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
+            if (c->kernel_ms < shape_ms) {
+                shape_ms = c->kernel_ms;
+                best_g = g;
+                best_ch = ch;
+            }
+        }
+        if (shape_ms < 0.98f * base_ms && (best_g != g0 || best_ch != ch0)) {
+            best.window_groups = best_g;
+            best.particles_per_chunk = best_ch;
         }
     }
     for (size_t id = 0; id < c->screens.size(); ++id) {

@@ -28,8 +28,9 @@ constexpr double kInvSpeedOfLight = 1.0 / kSpeedOfLight;  // RN(1/c), folded by 
 
 constexpr int kLanes = 64;  // one wavefront = one pixel tile
 
-// Packed trajectory point for the FP64 kernel: 12 doubles = 96 B, read by the
-// whole wave at one uniform address (scalar loads).  beta = u/gamma and
+// Packed trajectory point for the FP64 kernel: 12 doubles = 96 B, read per
+// lane (vector loads) by the per-lane walk, or staged through LDS for the
+// wave-uniform walk.  beta = u/gamma and
 // beta' = a/gamma are hoisted out of the pixel loop; both are computed exactly
 // as src/particle.cpp:575-582 computes them.
 struct alignas(32) RecF64 {
@@ -40,7 +41,8 @@ struct alignas(32) RecF64 {
     double scale;   // q e / (4 pi eps0)
 };
 
-// Packed trajectory point for the mixed kernel: 64 B, one scalar load.  FP64
+// Packed trajectory point for the mixed kernel: 64 B, read per lane with
+// vector loads in the per-lane walk, one scalar load in the wave-uniform walk.  FP64
 // where the retarded time and the dominant term of 1 - n.beta need it (t, X,
 // beta_z); FP32 for the transverse beta, beta' pre-multiplied by the charge
 // scale / c, and the velocity-term scale / gamma^2.  The FP32 transverse
