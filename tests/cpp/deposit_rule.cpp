@@ -3,7 +3,13 @@
 // (or_deposit_step), on random steps, including steps that start or end on
 // bucket edges, start before the trace or run past its end, and every split of
 // the trace into windows [b0, b1).  Prints the largest difference relative to
-// the largest trace value and exits non-zero above 1e-14.
+// the largest trace value and exits non-zero above 1e-14.  The mixed kernel's
+// bucket-unit form (deposit_step_units: index and FP32 offset from the bucket
+// centre, FP32 weights) is checked on the same steps in absolute terms: its
+// weights are bucket fractions good to ~1e-7 (FP32 offsets), so every bucket
+// must agree within 1e-6 of the larger end field; a wrong index, sign or
+// weight moves a step's share of a bucket, far above that for all but the
+// shortest steps drawn.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -26,6 +32,9 @@ struct Acc {
 struct Rcp {
     double operator()(double x) const { return 1.0 / x; }
 };
+struct RcpF {
+    float operator()(float x) const { return 1.0f / x; }
+};
 
 static double uni() { return rand() / (RAND_MAX + 1.0); }
 
@@ -33,7 +42,7 @@ int main(int argc, char** argv)
 {
     long cases = argc > 1 ? atol(argv[1]) : 200000;
     srand(12345);
-    double worst = 0.0;
+    double worst = 0.0, worst_units = 0.0;
     for (long it = 0; it < cases; it++) {
         int nb = 2 + rand() % 24;
         double dt = 0.05 + uni();
@@ -60,10 +69,26 @@ int main(int argc, char** argv)
             int b1 = b0 + W < nb ? b0 + W : nb;
             tg::deposit_step(acc, Rcp{}, true, ts1, i1, ts2, i2, t0, dt, inv, tmin, tmax, nb, b0, b1);
         }
-        double m = 0;
+        // bucket units: q = (ts - t0)/dt = idx + u, idx the nearest integer
+        // (half to even, as the kernel's rint), u the FP32 offset
+        std::vector<double> units(6 * nb, 0.0);
+        Acc acc_u{units.data(), f1, f2};
+        const double q1 = (ts1 - t0) / dt, q2 = (ts2 - t0) / dt;
+        const int j1 = (int)nearbyint(q1), j2 = (int)nearbyint(q2);
+        const float u1 = (float)(q1 - j1), u2 = (float)(q2 - j2);
+        for (int b0 = 0; b0 < nb; b0 += W) {
+            int b1 = b0 + W < nb ? b0 + W : nb;
+            tg::deposit_step_units<float>(acc_u, RcpF{}, Rcp{}, true, j1, u1, j2, u2, nb, b0, b1);
+        }
+        double m = 0, fm = 0;
         for (double v : ref) m = fmax(m, fabs(v));
-        for (int i = 0; i < 6 * nb; i++) worst = fmax(worst, fabs(got[i] - ref[i]) / (m > 0 ? m : 1.0));
+        for (int c = 0; c < 6; c++) fm = fmax(fm, fmax(fabs(f1[c]), fabs(f2[c])));
+        for (int i = 0; i < 6 * nb; i++) {
+            worst = fmax(worst, fabs(got[i] - ref[i]) / (m > 0 ? m : 1.0));
+            worst_units = fmax(worst_units, fabs(units[i] - ref[i]) / fm);
+        }
     }
-    printf("cases %ld worst relative difference %.3e\n", cases, worst);
-    return worst <= 1e-14 ? 0 : 1;
+    printf("cases %ld worst relative difference %.3e, bucket units (FP32 weights) %.3e of the end fields\n",
+           cases, worst, worst_units);
+    return worst <= 1e-14 && worst_units <= 1e-6 ? 0 : 1;
 }

@@ -112,7 +112,17 @@ struct PointT {
     F f[6];        // Ex, Ey, Ez, Bx, By, Bz
 };
 using Point = PointT<double>;
-using PointF = PointT<float>;
+
+// A trajectory point as the mixed kernel keeps it: the bucket position
+// q = (ts - t0)/dt as its bucket index idx = rint(q) and the FP32 offset
+// u = q - idx in [-1/2, 1/2] from that bucket's centre (the deposit needs
+// only these, step_common_units), and the amplitudes E and c B in FP32 (the
+// 1/c of B is applied when the LDS window is flushed into the FP64 trace).
+struct PointM {
+    int idx;
+    float u;
+    float f[6];    // Ex, Ey, Ez, c Bx, c By, c Bz
+};
 
 // Retarded time t + |P - X|/c and bucket index: src/particle.cpp:560-568,
 // 805, 829-830, in the reference's operation order.
@@ -167,13 +177,15 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
     o.f[5] = (nx * ey - ny * ex) * kInvSpeedOfLight;
 }
 
-// Retarded time and bucket index of the mixed kernel, also returning 1/R:
+// Retarded time and bucket position of the mixed kernel, also returning 1/R:
 // 1/sqrt from the FP32 hardware estimate and one FP64 Newton step (~2^-46
 // relative).  Every FMA is explicit, so every caller gets the same doubles.
-// The index rounds half to even, which differs from the reference's
-// half-away rounding only for a retarded time exactly on a bucket edge.
+// The position q = (ts - t0)/dt is returned as its nearest bucket idx (half
+// to even, which differs from the reference's half-away rounding only for a
+// retarded time exactly on a bucket edge; clamped to +-1e9) and the offset
+// u = q - idx, in [-1/2, 1/2] inside the clamp.
 __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, double dz, double t0,
-                                             double inv_dt, double& y, double& ts, int& idx)
+                                             double inv_dt, double& y, int& idx, float& u)
 {
 #pragma clang fp contract(off)
     const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
@@ -182,18 +194,21 @@ __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, dou
     const double e = __builtin_fma(-h, y, 1.0);
     y = __builtin_fma(0.5 * y, e, y);
     const double R = r2 * y;
-    ts = __builtin_fma(R, kInvSpeedOfLight, t);
-    idx = rint_index((ts - t0) * inv_dt);
+    const double ts = __builtin_fma(R, kInvSpeedOfLight, t);
+    const double q = (ts - t0) * inv_dt;
+    const double r = __builtin_fmin(__builtin_fmax(__builtin_rint(q), -1.0e9), 1.0e9);
+    idx = (int)r;
+    u = (float)(q - r);
 }
 
 // Mixed precision: FP64 distance, retarded time and the n_z beta_z part of
 // 1 - n.beta (the cancelling term), FP32 for everything else.
 __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double Py, double Pz,
-                                           double t0, double dt, double inv_dt, PointT<float>& o)
+                                           double t0, double dt, double inv_dt, PointM& o)
 {
     const double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
     double y;
-    timing_mixed(r.t, dx, dy, dz, t0, inv_dt, y, o.ts, o.idx);
+    timing_mixed(r.t, dx, dy, dz, t0, inv_dt, y, o.idx, o.u);
     const double nz = dz * y;
     const float fy = (float)y;
     const float nx = (float)dx * fy, ny = (float)dy * fy, fnz = (float)nz;
@@ -206,30 +221,29 @@ __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double 
     const float ex = mx * c1 - r.apx * c2;
     const float ey = my * c1 - r.apy * c2;
     const float ez = mz * c1 - r.apz * c2;
-    const float ic = (float)kInvSpeedOfLight;
     o.f[0] = ex;
     o.f[1] = ey;
     o.f[2] = ez;
-    o.f[3] = (ny * ez - fnz * ey) * ic;
-    o.f[4] = (fnz * ex - nx * ez) * ic;
-    o.f[5] = (nx * ey - ny * ex) * ic;
+    o.f[3] = ny * ez - fnz * ey;   // c B = n x E
+    o.f[4] = fnz * ex - nx * ez;
+    o.f[5] = nx * ey - ny * ex;
 }
 
-// Per precision: trajectory record, amplitude type (also the LDS window type).
-// FP64 accumulates the window in FP64 LDS; the mixed kernel accumulates an
-// FP32 LDS window that is flushed into the FP64 partial trace before its
-// cells hold much more than 2048 terms each.
+// Per precision: trajectory record, amplitude type (also the LDS window type)
+// and evaluated point.  FP64 accumulates the window in FP64 LDS; the mixed
+// kernel accumulates an FP32 LDS window that is flushed into the FP64 partial
+// trace before its cells hold much more than 2048 terms each.
 template <int PREC>
 struct RecOf;
 template <>
-struct RecOf<0> { using type = RecF64; using amp = double; };
+struct RecOf<0> { using type = RecF64; using amp = double; using point = Point; };
 template <>
-struct RecOf<1> { using type = RecMixed; using amp = float; };
+struct RecOf<1> { using type = RecMixed; using amp = float; using point = PointM; };
 
 template <int PREC>
 __device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, double Px, double Py,
                                            double Pz, double t0, double dt, double inv_dt,
-                                           PointT<typename RecOf<PREC>::amp>& o)
+                                           typename RecOf<PREC>::point& o)
 {
     if constexpr (PREC == 0)
         eval_f64(r, Px, Py, Pz, t0, dt, inv_dt, o);
@@ -244,12 +258,6 @@ struct RcpRn {
 struct RcpF32 {
     __device__ __forceinline__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
 };
-template <class F>
-struct RcpOf;
-template <>
-struct RcpOf<double> { using type = RcpRn; };
-template <>
-struct RcpOf<float> { using type = RcpF32; };
 
 // LDS trace window of one wave: cell (bucket j, component c, lane) at
 // ((j*6)+c)*64 + lane, W buckets plus one per-lane trash bucket (W) that
@@ -261,7 +269,8 @@ template <class F>
 struct Window {
     using LdsF = __attribute__((address_space(3))) F;
     LdsF* lane_base;  // lds + lane
-    __device__ __forceinline__ LdsF* cell(int j) const { return lane_base + j * (6 * kLanes); }
+    // j in [0, W]: a 24-bit product (one full-rate v_mad_u32_u24 for the address)
+    __device__ __forceinline__ LdsF* cell(int j) const { return lane_base + __mul24(j, 6 * kLanes); }
     // *p += v.  FP64 cells add in the LDS unit (ds_add_f64): each lane owns
     // its column, so no two lanes meet at one address and the adds of a lane
     // happen in program order; the VALU does not read the cell back.  FP32
@@ -332,24 +341,46 @@ __device__ __forceinline__ int wave_min(int v)
     return v;
 }
 
-// Deposit of step (p1 -> p2) into the window [b0, b1).
-template <class F>
-__device__ __forceinline__ void deposit_window(const Window<F>& win, int W, bool active, const PointT<F>& p1,
-                                               const PointT<F>& p2, double t0, double dt, double inv_dt,
-                                               double tmin, double tmax, int n_bins, int b0, int b1)
+// Deposit of step (p1 -> p2) into the window [b0, b1), FP64: retarded times
+// and bucket edges as the reference forms them.
+__device__ __forceinline__ void deposit_window(const Window<double>& win, int W, bool active, const Point& p1,
+                                               const Point& p2, double t0, double dt, double inv_dt, double tmin,
+                                               double tmax, int n_bins, int b0, int b1)
 {
     const bool valid = active && p2.ts > tmin && p1.ts < tmax;    // particle.cpp:793
     const bool case_a = p1.ts <= tmin;
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
-    const TwoBuckets<F> t = step_common<F>(typename RcpOf<F>::type{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2,
-                                           t0, dt, inv_dt, n_bins, b0, b1, W, W);
+    const TwoBuckets<double> t = step_common<double>(RcpRn{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
+                                                     inv_dt, n_bins, b0, b1, W, W);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
-            WindowAcc<F> acc{win, b0, p1.f, p2.f};
+            WindowAcc<double> acc{win, b0, p1.f, p2.f};
             step_general(acc, rcp_rn(p2.ts - p1.ts), case_a, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt, n_bins,
                          b0, b1);
+        }
+    }
+}
+
+// Mixed: the same rule in bucket units (deposit_step_units), the common
+// step branch-free with FP32 weights, the general step under a ballot.
+__device__ __forceinline__ void deposit_window(const Window<float>& win, int W, bool active, const PointM& p1,
+                                               const PointM& p2, double, double, double, double, double,
+                                               int n_bins, int b0, int b1)
+{
+    const bool valid = active && p2.idx >= 0 && p1.idx < n_bins;  // particle.cpp:793
+    const bool case_a = p1.idx < 0;                               // particle.cpp:796
+    const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
+    const bool gen = valid && step_is_general(case_a, p1.idx, i2);
+    const TwoBuckets<float> t =
+        step_common_units<float>(RcpF32{}, valid && !gen, p1.idx, p1.u, p2.idx, p2.u, i2, b0, b1, W, W);
+    win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
+    if (__ballot(gen)) {
+        if (gen) {
+            WindowAcc<float> acc{win, b0, p1.f, p2.f};
+            const double q1 = (double)p1.idx + (double)p1.u, q2 = (double)p2.idx + (double)p2.u;
+            step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0, b1);
         }
     }
 }
@@ -466,9 +497,9 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
                                              const Window<typename RecOf<PREC>::amp>& win, double* stage,
                                              int lane, bool verify, int& nk, int& nmin)
 {
-    using F = typename RecOf<PREC>::amp;
+    using Pt = typename RecOf<PREC>::point;
     auto src = make_source<PREC>(rp, NP, stage, lane);
-    PointT<F> p0, p1;
+    Pt p0, p1;
     src.start(k);
     eval_point<PREC>(src.first(k), Px, Py, Pz, t0, dt, inv_dt, p0);
     if (verify) {
@@ -492,7 +523,7 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
             break;
         }
         eval_point<PREC>(src.next(k + 1), Px, Py, Pz, t0, dt, inv_dt, p1);
-        deposit_window<F>(win, W, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        deposit_window(win, W, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
         if (next_k < 0 && __ballot(active && p1.idx >= b1)) {
             next_k = k;
@@ -526,8 +557,9 @@ __device__ __forceinline__ int point_index(const typename RecOf<PREC>::type* __r
         double R, ts;
         timing_exact(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, dt, inv_dt, R, ts, idx);
     } else {
-        double y, ts;
-        timing_mixed(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, inv_dt, y, ts, idx);
+        double y;
+        float u;
+        timing_mixed(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, inv_dt, y, idx, u);
     }
     return idx;
 }
@@ -557,7 +589,6 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
                                           const unsigned char* __restrict__ lc_in, unsigned char* __restrict__ lc_out,
                                           int& nk, int& nmin)
 {
-    using F = typename RecOf<PREC>::amp;
     using Rec = typename RecOf<PREC>::type;
     int kl = k;
     if (search) {
@@ -588,7 +619,8 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
             }
         }
     }
-    PointT<F> p0;
+    using Pt = typename RecOf<PREC>::point;
+    Pt p0;
     eval_point<PREC>(rp[kl], Px, Py, Pz, t0, dt, inv_dt, p0);
     bool walking = active && kl < NP - 1 && p0.idx < b1;
     int steps = 0;
@@ -601,58 +633,40 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     // window.
     int idx0 = p0.idx;
     Rec ra = rp[kl + 1 < NP ? kl + 1 : NP - 1];
-    if constexpr (PREC == 0) {
-        // FP64: two steps per loop trip, with the roles of the two point
-        // buffers and the two record buffers swapped between them, so the
-        // step's end point is not copied into the next step's start point,
-        // nor the read-ahead record into the current one (6 % faster on c2;
-        // the mixed kernel gains nothing from it).
-        PointT<F> pb;
-        Rec rb;
-        auto step = [&](const Rec& rc, Rec& rnext, const PointT<F>& ps, PointT<F>& pe) {
-            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-            __builtin_amdgcn_sched_barrier(0);
-            rnext = rp[kl + 2 < NP ? kl + 2 : NP - 1];
-            eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
-            deposit_window<F>(win, W, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
-            ++steps;
-            if (walking) {
-                if (pe.idx >= b1) {
-                    walking = false;
-                } else {
-                    ++kl;
-                    idx0 = pe.idx;
-                    walking = kl < NP - 1;
-                }
+    // The read-ahead of record kl + 2 is addressed as a 32-bit byte offset
+    // from the particle's records (NP records, far below 4 GB) and is not
+    // clamped to the particle: kl <= NP - 1, so it names at most two records
+    // past it, which exist (kRecPad) and whose contents only stopped lanes
+    // evaluate, without depositing.
+    const char* const rbase = reinterpret_cast<const char*>(rp);
+    // Two steps per loop trip, with the roles of the two point buffers and
+    // the two record buffers swapped between them, so the step's end point is
+    // not copied into the next step's start point, nor the read-ahead record
+    // into the current one (on c2 6 % faster for FP64, 1 % for mixed).
+    Pt pb;
+    Rec rb;
+    auto step = [&](const Rec& rc, Rec& rnext, const Pt& ps, Pt& pe) {
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __builtin_amdgcn_sched_barrier(0);
+        rnext = *reinterpret_cast<const Rec*>(rbase + (unsigned)(kl + 2) * (unsigned)sizeof(Rec));
+        eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
+        deposit_window(win, W, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        ++steps;
+        if (walking) {
+            if (pe.idx >= b1) {
+                walking = false;
+            } else {
+                ++kl;
+                idx0 = pe.idx;
+                walking = kl < NP - 1;
             }
-        };
-        for (;;) {
-            if (!__ballot(walking)) break;
-            step(ra, rb, p0, pb);
-            if (!__ballot(walking)) break;
-            step(rb, ra, pb, p0);
         }
-    } else {
-        PointT<F> p1;
-        while (__ballot(walking)) {
-            __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-            __builtin_amdgcn_sched_barrier(0);
-            const Rec rn = rp[kl + 2 < NP ? kl + 2 : NP - 1];
-            eval_point<PREC>(ra, Px, Py, Pz, t0, dt, inv_dt, p1);
-            deposit_window<F>(win, W, walking, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
-            ++steps;
-            if (walking) {
-                if (p1.idx >= b1) {
-                    walking = false;
-                } else {
-                    ++kl;
-                    p0 = p1;
-                    idx0 = p1.idx;
-                    walking = kl < NP - 1;
-                }
-            }
-            ra = rn;
-        }
+    };
+    for (;;) {
+        if (!__ballot(walking)) break;
+        step(ra, rb, p0, pb);
+        if (!__ballot(walking)) break;
+        step(rb, ra, pb, p0);
     }
     const bool stopped = active && kl < NP - 1;
     const int k_next = wave_min(stopped ? kl : INT_MAX);
@@ -742,12 +756,17 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         bool first_flush = true;
         bool dirty = false;    // the window holds terms not yet in the partial trace
         unsigned terms = 0;    // estimated terms per cell since the last flush
+        // cell j of the window as an FP64 trace value (mixed cells hold c B
+        // for the magnetic components)
+        auto cell = [&](int j, int c) {
+            const double v = (double)win.lane_base[(6 * j + c) * kLanes];
+            return (PREC == 1 && c >= 3) ? v * kInvSpeedOfLight : v;
+        };
         auto flush = [&]() {
             if (active) {
-                for (int j = 0; j < ncell; ++j) {
-                    double v = (double)win.lane_base[j * kLanes];
-                    out[j] = first_flush ? v : out[j] + v;
-                }
+                for (int j = 0; j < b1 - b0; ++j)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) out[6 * j + c] = first_flush ? cell(j, c) : out[6 * j + c] + cell(j, c);
             }
             for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
             first_flush = false;
@@ -792,9 +811,13 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         }
         if (active) {
             if (PREC == 0 || first_flush) {
-                for (int j = 0; j < ncell; ++j) out[j] = (double)win.lane_base[j * kLanes];
+                for (int j = 0; j < b1 - b0; ++j)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) out[6 * j + c] = cell(j, c);
             } else if (dirty) {
-                for (int j = 0; j < ncell; ++j) out[j] += (double)win.lane_base[j * kLanes];
+                for (int j = 0; j < b1 - b0; ++j)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) out[6 * j + c] += cell(j, c);
             }
         }
     }

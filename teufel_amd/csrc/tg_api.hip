@@ -191,8 +191,11 @@ int ensure_records(tg_context* c, int prec)
 {
     if (c->rec_valid[prec]) return TG_OK;
     size_t n = (size_t)c->n_particles * c->n_points;
-    if (!c->d_rec[prec])
-        TG_HIP(c, hipMalloc(&c->d_rec[prec], std::max<size_t>(n, 1) * (prec == TG_FP64 ? sizeof(RecF64) : sizeof(RecMixed))));
+    if (!c->d_rec[prec]) {
+        const size_t rec = prec == TG_FP64 ? sizeof(RecF64) : sizeof(RecMixed);
+        TG_HIP(c, hipMalloc(&c->d_rec[prec], (n + kRecPad) * rec));
+        TG_HIP(c, hipMemsetAsync((char*)c->d_rec[prec] + n * rec, 0, kRecPad * rec, c->stream));
+    }
     if (n) {
         unsigned blocks = (unsigned)((n + 255) / 256);
         if (prec == TG_FP64)
@@ -1157,6 +1160,9 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         saved[id].resize(nt);
         TG_HIP(c, hipMemcpy(saved[id].data(), s.d_traces, nt * sizeof(double), hipMemcpyDeviceToHost));
     }
+    // TG_AUTOTUNE_LOG=1 prints every trial's kernel time to stderr
+    const char* log_env = std::getenv("TG_AUTOTUNE_LOG");
+    const bool log = log_env && log_env[0] == '1';
     float best_ms = 1e30f;
     int best_w = default_window(c, max_bins);
     LaunchPlan best_full;
@@ -1169,6 +1175,9 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         t.particles_per_chunk = full.chunk;
         t.window_groups = full.groups;
         if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
+        if (log)
+            std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms (%d particles)\n", w, full.groups,
+                         full.chunk, c->kernel_ms, sample);
         if (c->kernel_ms < best_ms) {
             best_ms = c->kernel_ms;
             best_w = w;
@@ -1198,6 +1207,8 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
             t.particles_per_chunk = ch;
             t.window_groups = g;
             if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
+            if (log)
+                std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms\n", best_w, g, ch, c->kernel_ms);
             if (c->kernel_ms < shape_ms) {
                 shape_ms = c->kernel_ms;
                 best_g = g;

@@ -71,6 +71,34 @@ TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, 
     return o;
 }
 
+// step_common in bucket units, for the mixed kernel: a point sits at bucket
+// position q = (ts - t0)/dt = idx + u, u in [-1/2, 1/2] its offset from the
+// centre of its bucket idx, so the edge e between buckets idx1 and idx1 + 1
+// lies at offset 1/2 of idx1 and the weights need only the two offsets:
+//   (e - ts1)/dt = 1/2 - u1,  (ts2 - e)/dt = (idx2 - idx1) + u2 - 1/2
+// (for a single-bucket step e = ts2: u2 - u1 and 0).  idx2 is the unclamped
+// index of ts2 (i2 = min(idx2, n_bins)); [b0, b1) lies within the trace.
+template <class F, class Rcp>
+TG_HD inline TwoBuckets<F> step_common_units(const Rcp& rcp, bool common, int idx1, F u1, int idx2, F u2, int i2,
+                                             int b0, int b1, int none_a, int none_b)
+{
+    TwoBuckets<F> o;
+    const bool single = i2 == idx1;
+    const F da = (single ? u2 : (F)0.5) - u1;
+    const F db = single ? (F)0 : (F)(idx2 - idx1) + (u2 - (F)0.5);
+    const F idts = rcp(single ? (F)1 : da + db);
+    const F la = single ? (F)1 : da * idts, lb = single ? (F)0 : db * idts;
+    const F ha = (F)0.5 * da, hb = (F)0.5 * db;
+    o.aa = ha * ((F)1 + lb);
+    o.ba = ha * la;
+    o.ab = hb * lb;
+    o.bb = hb * (la + (F)1);
+    const unsigned span = (unsigned)(b1 - b0);
+    o.ja = (common && (unsigned)(idx1 - b0) < span) ? idx1 - b0 : none_a;
+    o.jb = (common && !single && (unsigned)(i2 - b0) < span) ? i2 - b0 : none_b;
+    return o;
+}
+
 // One bucket j of a general step.  first: the step starts in bucket j (and not
 // before the trace); last: the step ends in bucket j (j < n_bins).
 template <class Acc>
@@ -119,6 +147,27 @@ TG_HD inline void deposit_step(const Acc& acc, const Rcp& rcp, bool active, doub
     }
     const TwoBuckets<double> t =
         step_common<double>(rcp, true, ts1, idx1, ts2, i2, t0, dt, inv_dt, n_bins, b0, b1, -1, -1);
+    if (t.ja >= 0) acc(t.ja + b0, t.aa, t.ba);
+    if (t.jb >= 0) acc(t.jb + b0, t.ab, t.bb);
+}
+
+// deposit_step in bucket units (the mixed kernel's form): points at
+// q = idx + u (step_common_units); the trace spans q in (-1/2, n_bins - 1/2),
+// so the window test and case A become index tests, and a general step runs
+// step_general with t0 = 0, dt = 1.  rcp: reciprocal in F, rcpd in double.
+template <class F, class Acc, class Rcp, class RcpD>
+TG_HD inline void deposit_step_units(const Acc& acc, const Rcp& rcp, const RcpD& rcpd, bool active, int idx1, F u1,
+                                     int idx2, F u2, int n_bins, int b0, int b1)
+{
+    if (!(active && idx2 >= 0 && idx1 < n_bins)) return;         // particle.cpp:793
+    const bool case_a = idx1 < 0;                               // particle.cpp:796
+    const int i2 = idx2 > n_bins ? n_bins : idx2;
+    if (step_is_general(case_a, idx1, i2)) {
+        const double q1 = (double)idx1 + (double)u1, q2 = (double)idx2 + (double)u2;
+        step_general(acc, rcpd(q2 - q1), case_a, q1, idx1, q2, i2, 0.0, 1.0, 1.0, n_bins, b0, b1);
+        return;
+    }
+    const TwoBuckets<F> t = step_common_units<F>(rcp, true, idx1, u1, idx2, u2, i2, b0, b1, -1, -1);
     if (t.ja >= 0) acc(t.ja + b0, t.aa, t.ba);
     if (t.jb >= 0) acc(t.jb + b0, t.ab, t.bb);
 }
