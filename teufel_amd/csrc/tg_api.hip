@@ -1135,14 +1135,19 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // they are launched: every wave of a launch gets the LDS of the largest
     // window, so one window for the whole batch is what occupancy depends on.
     // The sample is sized to about 1e11 (pixel, step) pairs per candidate
-    // (under a second), and it runs with the chunk size and window groups of
-    // the full launch, so its work items have the full launch's shape.
+    // (under a second).  It runs with the window groups of the full launch
+    // and is cut into as many particle chunks as the full launch, so it has
+    // the full launch's number of work items per resident wave (which sets
+    // how long the launch's tail is); only the items' particle count is
+    // smaller.
     const int cands[] = {4, 5, 6, 8, 10, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     double pairs_per_particle = 0;
     for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
     const double want = 1e11 / std::max(1.0, pairs_per_particle);
     int sample = (int)std::min<double>(c->n_particles, std::max(64.0, want));
+    // particles per chunk of the sample when the full launch has n_ch chunks
+    auto sample_chunk = [&](long long n_ch) { return (int)std::max<long long>(1, (sample + n_ch - 1) / n_ch); };
     std::vector<int> ids;
     for (int i = 0; i < (int)c->screens.size(); ++i)
         if (c->screens[i].n_bins >= 2 && c->screens[i].n_pix > 0) ids.push_back(i);
@@ -1172,7 +1177,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         t.bins_per_window = w;
         LaunchPlan full;
         if ((rc = make_plan(c, precision, ids, 0, c->n_particles, &t, full))) return rc;
-        t.particles_per_chunk = full.chunk;
+        t.particles_per_chunk = sample_chunk(full.n_chunks);
         t.window_groups = full.groups;
         if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
         if (log)
@@ -1204,7 +1209,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
             if (n_ch * per_chunk * sizeof(double) > ((size_t)16 << 30)) continue;
             tg_tuning t = {};
             t.bins_per_window = best_w;
-            t.particles_per_chunk = ch;
+            t.particles_per_chunk = sample_chunk((long long)n_ch);
             t.window_groups = g;
             if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
             if (log)
