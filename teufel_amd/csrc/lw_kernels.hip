@@ -331,6 +331,76 @@ struct WindowAcc {
 // vmcnt[3:0] in bits 3:0 and [5:4] in bits 15:14, expcnt in 6:4, lgkmcnt in 11:8).
 constexpr int kWaitVmcnt0 = 0x0F70;
 
+// The records of one particle as a buffer resource (wave-uniform): the
+// per-lane walk reads a record as whole 16-B buffer loads (4 for the mixed
+// record, 6 for FP64) and takes the fields by bit casts.  The gathers of the
+// per-lane walk are bound by the vector-memory address path, which handles
+// every load instruction separately, and a field-typed copy of the record
+// compiles to more, narrower loads.  The resource covers the particle's NP
+// records: a read past them (the walk's read-ahead near the end of a
+// particle) returns zeros, which only stopped lanes evaluate.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBufferWord3 = 0x00020000;   // raw 32-bit data format, gfx9 buffer resource word 3
+
+__device__ __forceinline__ double dbl(unsigned lo, unsigned hi)
+{
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float flt(unsigned v) { return __builtin_bit_cast(float, v); }
+
+template <class Rec>
+struct RecBuffer {
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ __forceinline__ RecBuffer(const Rec* rp, int NP)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<Rec*>(rp), (short)0, NP * (int)sizeof(Rec),
+                                                 kBufferWord3))
+    {
+    }
+    __device__ __forceinline__ u32x4 part(int k, int i) const
+    {
+        return __builtin_amdgcn_raw_buffer_load_b128(rsrc, k * (int)sizeof(Rec) + 16 * i, 0, 0);
+    }
+    __device__ __forceinline__ Rec load(int k) const;
+};
+
+template <>
+__device__ __forceinline__ RecMixed RecBuffer<RecMixed>::load(int k) const
+{
+    const u32x4 a = part(k, 0), b = part(k, 1), c = part(k, 2), d = part(k, 3);
+    RecMixed r;
+    r.t = dbl(a.x, a.y);
+    r.x = dbl(a.z, a.w);
+    r.y = dbl(b.x, b.y);
+    r.z = dbl(b.z, b.w);
+    r.bz = dbl(c.x, c.y);
+    r.bx = flt(c.z);
+    r.by = flt(c.w);
+    r.apx = flt(d.x);
+    r.apy = flt(d.y);
+    r.apz = flt(d.z);
+    r.sv = flt(d.w);
+    return r;
+}
+template <>
+__device__ __forceinline__ RecF64 RecBuffer<RecF64>::load(int k) const
+{
+    const u32x4 a = part(k, 0), b = part(k, 1), c = part(k, 2), d = part(k, 3), e = part(k, 4), f = part(k, 5);
+    RecF64 r;
+    r.t = dbl(a.x, a.y);
+    r.x = dbl(a.z, a.w);
+    r.y = dbl(b.x, b.y);
+    r.z = dbl(b.z, b.w);
+    r.bx = dbl(c.x, c.y);
+    r.by = dbl(c.z, c.w);
+    r.bz = dbl(d.x, d.y);
+    r.bpx = dbl(d.z, d.w);
+    r.bpy = dbl(e.x, e.y);
+    r.bpz = dbl(e.z, e.w);
+    r.g2inv = dbl(f.x, f.y);
+    r.scale = dbl(f.z, f.w);
+    return r;
+}
+
 __device__ __forceinline__ int wave_min(int v)
 {
 #pragma unroll
@@ -632,13 +702,10 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     // lane's current step start, which a stopped lane reports for the next
     // window.
     int idx0 = p0.idx;
-    Rec ra = rp[kl + 1 < NP ? kl + 1 : NP - 1];
-    // The read-ahead of record kl + 2 is addressed as a 32-bit byte offset
-    // from the particle's records (NP records, far below 4 GB) and is not
-    // clamped to the particle: kl <= NP - 1, so it names at most two records
-    // past it, which exist (kRecPad) and whose contents only stopped lanes
-    // evaluate, without depositing.
-    const char* const rbase = reinterpret_cast<const char*>(rp);
+    // The read-ahead of record kl + 2 is not clamped to the particle: past
+    // its end the buffer resource returns zeros (kl <= NP - 1).
+    const RecBuffer<Rec> rbuf(rp, NP);
+    Rec ra = rbuf.load(kl + 1);
     // Two steps per loop trip, with the roles of the two point buffers and
     // the two record buffers swapped between them, so the step's end point is
     // not copied into the next step's start point, nor the read-ahead record
@@ -648,7 +715,7 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     auto step = [&](const Rec& rc, Rec& rnext, const Pt& ps, Pt& pe) {
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __builtin_amdgcn_sched_barrier(0);
-        rnext = *reinterpret_cast<const Rec*>(rbase + (unsigned)(kl + 2) * (unsigned)sizeof(Rec));
+        rnext = rbuf.load(kl + 2);
         eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
         deposit_window(win, W, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;

@@ -175,6 +175,10 @@ double* raw_charge(tg_context* c) { return c->d_raw + (size_t)kNumRaw * c->n_par
 
 int alloc_traj(tg_context* c, int n_particles, int n_points)
 {
+    // the radiation kernel addresses one particle's packed records (96 B per
+    // point in FP64) with 32-bit signed byte offsets
+    if ((long long)n_points * (long long)sizeof(RecF64) > (long long)INT_MAX)
+        return fail(c, TG_ERR_ARG, "more than 22 million points per particle");
     int rc = free_traj(c);
     if (rc) return rc;
     size_t n = (size_t)n_particles * (size_t)n_points;
@@ -191,11 +195,8 @@ int ensure_records(tg_context* c, int prec)
 {
     if (c->rec_valid[prec]) return TG_OK;
     size_t n = (size_t)c->n_particles * c->n_points;
-    if (!c->d_rec[prec]) {
-        const size_t rec = prec == TG_FP64 ? sizeof(RecF64) : sizeof(RecMixed);
-        TG_HIP(c, hipMalloc(&c->d_rec[prec], (n + kRecPad) * rec));
-        TG_HIP(c, hipMemsetAsync((char*)c->d_rec[prec] + n * rec, 0, kRecPad * rec, c->stream));
-    }
+    if (!c->d_rec[prec])
+        TG_HIP(c, hipMalloc(&c->d_rec[prec], std::max<size_t>(n, 1) * (prec == TG_FP64 ? sizeof(RecF64) : sizeof(RecMixed))));
     if (n) {
         unsigned blocks = (unsigned)((n + 255) / 256);
         if (prec == TG_FP64)

@@ -56,10 +56,6 @@ struct alignas(64) RecMixed {
     float sv;              // scale / gamma^2
 };
 static_assert(sizeof(RecF64) == 96, "RecF64 layout");
-// Records allocated after the last particle's: the per-lane walk reads one
-// record ahead without clamping to its particle, up to two records past it
-// (the next particle's, or these), and never uses what it reads there.
-constexpr int kRecPad = 2;
 static_assert(sizeof(RecMixed) == 64, "RecMixed layout");
 
 // One screen as the kernel sees it.
