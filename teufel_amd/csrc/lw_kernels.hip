@@ -618,18 +618,19 @@ __device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* _
 // full evaluation of that precision computes it (t, x, y, z lead both record
 // types).
 template <int PREC>
-__device__ __forceinline__ int point_index(const typename RecOf<PREC>::type* __restrict__ rp, int k, double Px,
+__device__ __forceinline__ int point_index(const RecBuffer<typename RecOf<PREC>::type>& rb, int k, double Px,
                                            double Py, double Pz, double t0, double dt, double inv_dt)
 {
-    const typename RecOf<PREC>::type& r = rp[k];
+    const u32x4 a = rb.part(k, 0), b = rb.part(k, 1);
+    const double t = dbl(a.x, a.y), x = dbl(a.z, a.w), y = dbl(b.x, b.y), z = dbl(b.z, b.w);
     int idx;
     if constexpr (PREC == 0) {
         double R, ts;
-        timing_exact(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, dt, inv_dt, R, ts, idx);
+        timing_exact(t, Px - x, Py - y, Pz - z, t0, dt, inv_dt, R, ts, idx);
     } else {
-        double y;
+        double yr;
         float u;
-        timing_mixed(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, inv_dt, y, idx, u);
+        timing_mixed(t, Px - x, Py - y, Pz - z, t0, inv_dt, yr, idx, u);
     }
     return idx;
 }
@@ -660,6 +661,7 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
                                           int& nk, int& nmin)
 {
     using Rec = typename RecOf<PREC>::type;
+    const RecBuffer<Rec> rbuf(rp, NP);
     int kl = k;
     if (search) {
         int lo = 1, hi = NP;  // first point in [lo, hi) with bucket >= b0; NP: none
@@ -667,7 +669,7 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
         while (__ballot(open)) {
             if (open) {
                 const int mid = (lo + hi) >> 1;
-                if (point_index<PREC>(rp, mid, Px, Py, Pz, t0, dt, inv_dt) >= b0)
+                if (point_index<PREC>(rbuf, mid, Px, Py, Pz, t0, dt, inv_dt) >= b0)
                     hi = mid;
                 else
                     lo = mid + 1;
@@ -681,7 +683,7 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     bool skip = active && kl < NP - 1;
     while (__ballot(skip)) {
         if (skip) {
-            if (point_index<PREC>(rp, kl + 1, Px, Py, Pz, t0, dt, inv_dt) < b0) {
+            if (point_index<PREC>(rbuf, kl + 1, Px, Py, Pz, t0, dt, inv_dt) < b0) {
                 ++kl;
                 skip = kl < NP - 1;
             } else {
@@ -691,7 +693,7 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     }
     using Pt = typename RecOf<PREC>::point;
     Pt p0;
-    eval_point<PREC>(rp[kl], Px, Py, Pz, t0, dt, inv_dt, p0);
+    eval_point<PREC>(rbuf.load(kl), Px, Py, Pz, t0, dt, inv_dt, p0);
     bool walking = active && kl < NP - 1 && p0.idx < b1;
     int steps = 0;
     // Each step first waits for the record read ahead in the previous step,
@@ -704,7 +706,6 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     int idx0 = p0.idx;
     // The read-ahead of record kl + 2 is not clamped to the particle: past
     // its end the buffer resource returns zeros (kl <= NP - 1).
-    const RecBuffer<Rec> rbuf(rp, NP);
     Rec ra = rbuf.load(kl + 1);
     // Two steps per loop trip, with the roles of the two point buffers and
     // the two record buffers swapped between them, so the step's end point is
