@@ -34,13 +34,23 @@ namespace tg {
 // Packing: raw ChargedParticle trajectory (t, X, u=beta*gamma, a) -> records.
 // ---------------------------------------------------------------------------
 
+// Record of point k of particle p into the blocked layout (rec_offset).
+template <class Rec>
+__device__ __forceinline__ void store_rec(char* out, int n_points, size_t p, size_t k, const Rec& r)
+{
+    char* dst = out + p * rec_particle_bytes<Rec>(n_points) + rec_offset<Rec>((int)k);
+    const uint4* src = reinterpret_cast<const uint4*>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Rec) / 16); ++i) *reinterpret_cast<uint4*>(dst + kRecPieceStride<Rec> * i) = src[i];
+}
+
 __global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict__ t,
                                 const double* __restrict__ x, const double* __restrict__ y,
                                 const double* __restrict__ z, const double* __restrict__ ux,
                                 const double* __restrict__ uy, const double* __restrict__ uz,
                                 const double* __restrict__ ax, const double* __restrict__ ay,
                                 const double* __restrict__ az, const double* __restrict__ charge,
-                                RecF64* __restrict__ out)
+                                char* __restrict__ out)
 {
 #pragma clang fp contract(off)
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,7 +73,7 @@ __global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict
     r.bpz = az[g] / gamma;
     r.g2inv = 1.0 / gamma2;
     r.scale = charge[p] * kElementaryCharge / (4.0 * kPi * kEpsNull);  // particle.cpp:588
-    out[g] = r;
+    store_rec(out, n_points, p, g - p * (size_t)n_points, r);
 }
 
 __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restrict__ t,
@@ -72,7 +82,7 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
                                   const double* __restrict__ uy, const double* __restrict__ uz,
                                   const double* __restrict__ ax, const double* __restrict__ ay,
                                   const double* __restrict__ az, const double* __restrict__ charge,
-                                  RecMixed* __restrict__ out, int* __restrict__ off_axis)
+                                  char* __restrict__ out, int* __restrict__ off_axis)
 {
 #pragma clang fp contract(off)
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,7 +106,7 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
     r.apy = (float)(ay[g] / gamma * sa);
     r.apz = (float)(az[g] / gamma * sa);
     r.sv = (float)(scale / gamma2);
-    out[g] = r;
+    store_rec(out, n_points, p, g - p * (size_t)n_points, r);
     // the FP32 transverse beta needs |beta_x| + |beta_y| << 1 (see RecMixed)
     if (__builtin_fabs(bx) + __builtin_fabs(by) > 0.25) *off_axis = 1;
 }
@@ -336,9 +346,10 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 // record, 6 for FP64) and takes the fields by bit casts.  The gathers of the
 // per-lane walk are bound by the vector-memory address path, which handles
 // every load instruction separately, and a field-typed copy of the record
-// compiles to more, narrower loads.  The resource covers the particle's NP
-// records: a read past them (the walk's read-ahead near the end of a
-// particle) returns zeros, which only stopped lanes evaluate.
+// compiles to more, narrower loads.  The resource covers the particle's
+// records (in the blocked layout, rec_offset): a read past them (the walk's
+// read-ahead near the end of a particle) returns zeros, which only stopped
+// lanes evaluate.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kBufferWord3 = 0x00020000;   // raw 32-bit data format, gfx9 buffer resource word 3
 
@@ -351,14 +362,14 @@ __device__ __forceinline__ float flt(unsigned v) { return __builtin_bit_cast(flo
 template <class Rec>
 struct RecBuffer {
     __amdgpu_buffer_rsrc_t rsrc;
-    __device__ __forceinline__ RecBuffer(const Rec* rp, int NP)
-        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<Rec*>(rp), (short)0, NP * (int)sizeof(Rec),
+    __device__ __forceinline__ RecBuffer(const char* rp, int NP)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rp), (short)0, (int)rec_particle_bytes<Rec>(NP),
                                                  kBufferWord3))
     {
     }
     __device__ __forceinline__ u32x4 part(int k, int i) const
     {
-        return __builtin_amdgcn_raw_buffer_load_b128(rsrc, k * (int)sizeof(Rec) + 16 * i, 0, 0);
+        return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(rec_offset<Rec>(k) + kRecPieceStride<Rec> * i), 0, 0);
     }
     __device__ __forceinline__ Rec load(int k) const;
 };
@@ -469,10 +480,10 @@ constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
 
 template <class Rec>
 struct LdsStager {
-    const Rec* __restrict__ rp;
+    const char* __restrict__ rp;      // the particle's records (blocked layout; a chunk is 2 whole blocks)
     double* slots;                    // LDS: 2 chunks
     int lane;
-    int lim;                          // last legal 3-double group start (12*NP - 3)
+    int lim;                          // last legal 3-double group start in the particle's records
     int next_fill;                    // chunk held in (q0,q1,q2)
     double q0, q1, q2;
 
@@ -505,33 +516,57 @@ struct LdsStager {
         store(c0 + 1, d, e, f);
         next_fill = c0 + 2;
     }
+    // Record r of a staged chunk, gathered from its pieces (blocked layout).
+    __device__ __forceinline__ Rec at(int r) const
+    {
+        static_assert(kChunkRecs % kRecBlock<Rec> == 0, "a chunk is whole blocks");
+        constexpr int rd = (int)(sizeof(Rec) / sizeof(double));
+        const int j = r % kChunkRecs;
+        const double* b = slots + ((r / kChunkRecs) & 1) * kChunkDoubles + (j / kRecBlock<Rec>) * (kRecBlock<Rec> * rd) +
+                          (j % kRecBlock<Rec>) * 2;
+        Rec out;
+        double* o = reinterpret_cast<double*>(&out);
+#pragma unroll
+        for (int i = 0; i < rd / 2; ++i) {
+            o[2 * i] = b[(kRecPieceStride<Rec> / sizeof(double)) * i];
+            o[2 * i + 1] = b[(kRecPieceStride<Rec> / sizeof(double)) * i + 1];
+        }
+        return out;
+    }
     // Record r, r advancing by one per call after start.
-    __device__ __forceinline__ const Rec& next(int r)
+    __device__ __forceinline__ Rec next(int r)
     {
         if (r / kChunkRecs == next_fill) {
             store(next_fill, q0, q1, q2);
             load(next_fill + 1, q0, q1, q2);
             ++next_fill;
         }
-        return *reinterpret_cast<const Rec*>(slots + ((r / kChunkRecs) & 1) * kChunkDoubles + (r % kChunkRecs) * 12);
+        return at(r);
     }
-    __device__ __forceinline__ const Rec& first(int k) const
-    {
-        return *reinterpret_cast<const Rec*>(slots + ((k / kChunkRecs) & 1) * kChunkDoubles + (k % kChunkRecs) * 12);
-    }
+    __device__ __forceinline__ Rec first(int k) const { return at(k); }
 };
 
 template <class Rec>
 struct ScalarPrefetch {
-    const Rec* __restrict__ rp;
+    const char* __restrict__ rp;   // the particle's records (blocked layout)
     int NP;
     Rec rn;   // record of the next point
-    __device__ __forceinline__ void start(int k) { rn = rp[k + 1]; }
-    __device__ __forceinline__ Rec first(int k) const { return rp[k]; }
+    // record k (uniform), gathered from its pieces
+    __device__ __forceinline__ Rec at(int k) const
+    {
+        Rec r;
+        uint4* o = reinterpret_cast<uint4*>(&r);
+        const char* b = rp + rec_offset<Rec>(k);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(Rec) / 16); ++i) o[i] = *reinterpret_cast<const uint4*>(b + kRecPieceStride<Rec> * i);
+        return r;
+    }
+    __device__ __forceinline__ void start(int k) { rn = at(k + 1); }
+    __device__ __forceinline__ Rec first(int k) const { return at(k); }
     __device__ __forceinline__ Rec next(int r)
     {
         const Rec cur = rn;
-        rn = rp[r + 1 < NP ? r + 1 : NP - 1];
+        rn = at(r + 1 < NP ? r + 1 : NP - 1);
         return cur;
     }
 };
@@ -544,11 +579,11 @@ template <>
 struct SourceOf<1> { using type = ScalarPrefetch<RecMixed>; static constexpr int lds_doubles = 0; };
 
 template <int PREC>
-__device__ __forceinline__ typename SourceOf<PREC>::type make_source(const typename RecOf<PREC>::type* rp, int NP,
-                                                                     double* stage, int lane)
+__device__ __forceinline__ typename SourceOf<PREC>::type make_source(const char* rp, int NP, double* stage, int lane)
 {
     if constexpr (PREC == 0)
-        return LdsStager<RecF64>{rp, stage, lane, 12 * NP - 3, 0, 0.0, 0.0, 0.0};
+        return LdsStager<RecF64>{rp, stage, lane, (int)(rec_particle_bytes<RecF64>(NP) / sizeof(double)) - 3, 0, 0.0,
+                                 0.0, 0.0};
     else
         return ScalarPrefetch<RecMixed>{rp, NP, RecMixed{}};
 }
@@ -560,7 +595,7 @@ __device__ __forceinline__ typename SourceOf<PREC>::type make_source(const typen
 // load one step ahead of its use.  With verify, k is a searched guess: step
 // back until no lane starts at or beyond b0 at point k.
 template <int PREC>
-__device__ __forceinline__ int walk_particle(const typename RecOf<PREC>::type* __restrict__ rp, int k,
+__device__ __forceinline__ int walk_particle(const char* __restrict__ rp, int k,
                                              int NP, bool active, double Px, double Py, double Pz,
                                              double t0, double dt, double inv_dt, double tmin,
                                              double tmax, int n_bins, int b0, int b1, int W,
@@ -653,7 +688,7 @@ __device__ __forceinline__ int point_index(const RecBuffer<typename RecOf<PREC>:
 // lc_out (when not null) receives the offsets of this walk's stopping steps
 // from nk.
 template <int PREC>
-__device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __restrict__ rp, int k, int NP,
+__device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, int NP,
                                           bool active, double Px, double Py, double Pz, double t0, double dt,
                                           double inv_dt, double tmin, double tmax, int n_bins, int b0, int b1,
                                           int W, const Window<typename RecOf<PREC>::amp>& win, bool search,
@@ -744,21 +779,31 @@ __device__ __forceinline__ int walk_lanes(const typename RecOf<PREC>::type* __re
     return steps;
 }
 
+// The records of particle p (blocked layout).
+template <int PREC>
+__device__ __forceinline__ const char* particle_records(const typename RecOf<PREC>::type* recs, int p, int NP)
+{
+    return reinterpret_cast<const char*>(recs) + (size_t)p * rec_particle_bytes<typename RecOf<PREC>::type>(NP);
+}
+
 // First trajectory point of each particle of a batch whose bucket for one
 // representative pixel is >= b: lanes = particles, binary search over the
 // monotone retarded time.  The result seeds a cursor that the walk verifies.
 template <int PREC>
-__device__ __forceinline__ int search_first_point(const typename RecOf<PREC>::type* __restrict__ rp_lane, int NP,
-                                                  double Px, double Py, double Pz, double t0, double dt,
-                                                  double inv_dt, int b)
+__device__ __forceinline__ int search_first_point(const char* __restrict__ rp_lane, int NP, double Px, double Py,
+                                                  double Pz, double t0, double dt, double inv_dt, int b)
 {
+    using Rec = typename RecOf<PREC>::type;
     int lo = 0, hi = NP - 1;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        const auto& r = rp_lane[mid];
+        // t, x, y, z: pieces 0 and 1 of both record types
+        const char* p = rp_lane + rec_offset<Rec>(mid);
+        const uint4 a = *reinterpret_cast<const uint4*>(p), c = *reinterpret_cast<const uint4*>(p + kRecPieceStride<Rec>);
+        const double t = dbl(a.x, a.y), x = dbl(a.z, a.w), y = dbl(c.x, c.y), z = dbl(c.z, c.w);
         double R, ts;
         int idx;
-        timing_exact(r.t, Px - r.x, Py - r.y, Pz - r.z, t0, dt, inv_dt, R, ts, idx);
+        timing_exact(t, Px - x, Py - y, Pz - z, t0, dt, inv_dt, R, ts, idx);
         if (idx >= b)
             hi = mid;
         else
@@ -810,7 +855,8 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         for (int pb = it.p0; pb < it.p1; pb += kLanes) {
             const int p = pb + lane;
             if (p < it.p1) {
-                int j = search_first_point<PREC>(recs + (size_t)p * NP, NP, Rx, Ry, Rz, Rt0, dt, inv_dt, it.bin_lo);
+                int j = search_first_point<PREC>(particle_records<PREC>(recs, p, NP), NP, Rx, Ry, Rz, Rt0, dt, inv_dt,
+                                                 it.bin_lo);
                 int k = j > 0 ? j - 1 : 0;
                 cur_base[p - it.p0] = make_int2(k < NP - 1 ? k : NP - 1, INT_MIN);
             }
@@ -858,12 +904,12 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                     // per-lane offsets: read when a previous window of this
                     // item stored them, written when a next window follows
                     unsigned char* lc = lane_cur ? lane_cur + (size_t)(p - it.p0) * kLanes : nullptr;
-                    ws = walk_lanes<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin, tmax,
+                    ws = walk_lanes<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin, tmax,
                                           nb, b0, b1, W, win, first, first ? nullptr : lc,
                                           b1 < it.bin_hi ? lc : nullptr, nk, nmin);
                 } else {
                     const bool verify = first && k > 0;
-                    ws = walk_particle<PREC>(recs + (size_t)p * NP, k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin,
+                    ws = walk_particle<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin,
                                              tmax, nb, b0, b1, W, win, stage, lane, verify, nk, nmin);
                 }
                 n_steps += ws;

@@ -177,7 +177,7 @@ int alloc_traj(tg_context* c, int n_particles, int n_points)
 {
     // the radiation kernel addresses one particle's packed records (96 B per
     // point in FP64) with 32-bit signed byte offsets
-    if ((long long)n_points * (long long)sizeof(RecF64) > (long long)INT_MAX)
+    if ((long long)rec_particle_bytes<RecF64>(n_points) > (long long)INT_MAX)
         return fail(c, TG_ERR_ARG, "more than 22 million points per particle");
     int rc = free_traj(c);
     if (rc) return rc;
@@ -195,15 +195,22 @@ int ensure_records(tg_context* c, int prec)
 {
     if (c->rec_valid[prec]) return TG_OK;
     size_t n = (size_t)c->n_particles * c->n_points;
-    if (!c->d_rec[prec])
-        TG_HIP(c, hipMalloc(&c->d_rec[prec], std::max<size_t>(n, 1) * (prec == TG_FP64 ? sizeof(RecF64) : sizeof(RecMixed))));
+    if (!c->d_rec[prec]) {
+        // blocked layout (rec_offset): whole blocks per particle, the padding
+        // points zero (the pack kernels write only real points)
+        const size_t bytes = std::max<size_t>(1, (size_t)c->n_particles * (prec == TG_FP64
+                                                                               ? rec_particle_bytes<RecF64>(c->n_points)
+                                                                               : rec_particle_bytes<RecMixed>(c->n_points)));
+        TG_HIP(c, hipMalloc(&c->d_rec[prec], bytes));
+        TG_HIP(c, hipMemsetAsync(c->d_rec[prec], 0, bytes, c->stream));
+    }
     if (n) {
         unsigned blocks = (unsigned)((n + 255) / 256);
         if (prec == TG_FP64)
             pack_f64_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
                                                          raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
                                                          raw(c, 8), raw(c, 9), raw_charge(c),
-                                                         (RecF64*)c->d_rec[prec]);
+                                                         (char*)c->d_rec[prec]);
         else
         {
             if (!c->d_flag) TG_HIP(c, hipMalloc((void**)&c->d_flag, sizeof(int)));
@@ -211,7 +218,7 @@ int ensure_records(tg_context* c, int prec)
             pack_mixed_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
                                                            raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
                                                            raw(c, 8), raw(c, 9), raw_charge(c),
-                                                           (RecMixed*)c->d_rec[prec], c->d_flag);
+                                                           (char*)c->d_rec[prec], c->d_flag);
             TG_HIP(c, hipGetLastError());
             int off_axis = 0;
             TG_HIP(c, hipMemcpyAsync(&off_axis, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));

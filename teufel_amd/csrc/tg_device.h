@@ -58,6 +58,36 @@ struct alignas(64) RecMixed {
 static_assert(sizeof(RecF64) == 96, "RecF64 layout");
 static_assert(sizeof(RecMixed) == 64, "RecMixed layout");
 
+// Layout of the packed records in HBM.  Each particle's points are grouped
+// in blocks of kRecBlock<Rec> points; within a block the records are stored
+// piece-major: piece i (bytes [16 i, 16 i + 16) of the record) of the
+// block's points lie together.  The per-lane walk reads one piece of a
+// different point in every lane, so with blocks of 8 lanes at nearby points
+// share cache lines (8 points per 128-B line instead of 2).  That pays for
+// the mixed record (5 % on c2); for the FP64 record the extra address
+// arithmetic cost more than it saved (2 %), so its blocks are single points,
+// the plain array of records.  A particle's records take
+// rec_particle_bytes(NP) bytes, NP rounded up to whole blocks (the rest zero).
+template <class Rec>
+constexpr int kRecBlock = 1;
+template <>
+constexpr int kRecBlock<RecMixed> = 8;
+template <class Rec>
+__host__ __device__ constexpr size_t rec_particle_bytes(int NP)
+{
+    return (size_t)((NP + kRecBlock<Rec> - 1) / kRecBlock<Rec>) * kRecBlock<Rec> * sizeof(Rec);
+}
+// Byte offset of piece 0 of point k (k >= 0) in its particle's records;
+// piece i lies kRecPieceStride<Rec> i bytes further.
+template <class Rec>
+constexpr unsigned kRecPieceStride = 16u * kRecBlock<Rec>;
+template <class Rec>
+__host__ __device__ constexpr unsigned rec_offset(int k)
+{
+    return ((unsigned)k / kRecBlock<Rec>) * (unsigned)(kRecBlock<Rec> * sizeof(Rec)) +
+           ((unsigned)k % kRecBlock<Rec>) * 16u;
+}
+
 // One screen as the kernel sees it.
 struct ScreenDev {
     const double* obs;    // [n_pix][3]
