@@ -225,8 +225,8 @@ __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double 
     const float kappa = (float)__builtin_fma(-nz, r.bz, 1.0) - (nx * r.bx + ny * r.by);
     const float mx = nx - r.bx, my = ny - r.by, mz = (float)(nz - r.bz);
     const float irk3 = fy * __builtin_amdgcn_rcpf(kappa * kappa * kappa);   // 1/(R kappa^3)
-    const float cv = r.sv * fy * irk3;                                       // scale/(R^2 kappa^3 gamma^2)
-    const float c1 = cv + irk3 * (nx * r.apx + ny * r.apy + fnz * r.apz);
+    // scale/(R^2 kappa^3 gamma^2) + n.beta' scale/(R kappa^3 c)
+    const float c1 = irk3 * __builtin_fmaf(r.sv, fy, nx * r.apx + ny * r.apy + fnz * r.apz);
     const float c2 = irk3 * kappa;
     const float ex = mx * c1 - r.apx * c2;
     const float ey = my * c1 - r.apy * c2;

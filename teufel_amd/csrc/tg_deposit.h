@@ -73,21 +73,27 @@ TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, 
 
 // step_common in bucket units, for the mixed kernel: a point sits at bucket
 // position q = (ts - t0)/dt = idx + u, u in [-1/2, 1/2] its offset from the
-// centre of its bucket idx, so the edge e between buckets idx1 and idx1 + 1
-// lies at offset 1/2 of idx1 and the weights need only the two offsets:
-//   (e - ts1)/dt = 1/2 - u1,  (ts2 - e)/dt = (idx2 - idx1) + u2 - 1/2
-// (for a single-bucket step e = ts2: u2 - u1 and 0).  idx2 is the unclamped
-// index of ts2 (i2 = min(idx2, n_bins)); [b0, b1) lies within the trace.
+// centre of its bucket idx.  Positions are taken relative to the centre of
+// bucket idx1: the step runs from u1 to x2 = (idx2 - idx1) + u2, the edge
+// between buckets idx1 and idx1 + 1 lies at 1/2 (a single-bucket step has
+// its "edge" at x2), so
+//   (e - ts1)/dt = e - u1,  (ts2 - e)/dt = x2 - e,  dts/dt = x2 - u1,
+// and no weight needs a select: a single-bucket step gets lb = 0 from
+// x2 - e = 0.  A zero-length step adds zeros (its length is raised to
+// 1e-30 before the reciprocal).  idx2 is the unclamped index of ts2
+// (i2 = min(idx2, n_bins)); [b0, b1) lies within the trace.
 template <class F, class Rcp>
 TG_HD inline TwoBuckets<F> step_common_units(const Rcp& rcp, bool common, int idx1, F u1, int idx2, F u2, int i2,
                                              int b0, int b1, int none_a, int none_b)
 {
     TwoBuckets<F> o;
     const bool single = i2 == idx1;
-    const F da = (single ? u2 : (F)0.5) - u1;
-    const F db = single ? (F)0 : (F)(idx2 - idx1) + (u2 - (F)0.5);
-    const F idts = rcp(single ? (F)1 : da + db);
-    const F la = single ? (F)1 : da * idts, lb = single ? (F)0 : db * idts;
+    const F x2 = (F)(idx2 - idx1) + u2;
+    const F e = single ? x2 : (F)0.5;
+    const F da = e - u1, db = x2 - e;
+    const F len = x2 - u1;
+    const F idts = rcp(len > (F)1e-30 ? len : (F)1e-30);
+    const F la = da * idts, lb = db * idts;
     const F ha = (F)0.5 * da, hb = (F)0.5 * db;
     o.aa = ha * ((F)1 + lb);
     o.ba = ha * la;
