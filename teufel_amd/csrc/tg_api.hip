@@ -1204,7 +1204,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // the full launch's number of work items per resident wave (which sets
     // how long the launch's tail is); only the items' particle count is
     // smaller.
-    const int cands[] = {4, 5, 6, 8, 10, 12, 16};
+    const int cands[] = {4, 5, 6, 7, 8, 9, 10, 12, 16};
     int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     double pairs_per_particle = 0;
     for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
