@@ -284,7 +284,7 @@ struct Window {
     // *p += v.  FP64 cells add in the LDS unit (ds_add_f64): each lane owns
     // its column, so no two lanes meet at one address and the adds of a lane
     // happen in program order; the VALU does not read the cell back.  FP32
-    // cells are read, added and written by the VALU (ds_add_f32 measured ten
+    // cells are read, added and written by the VALU (ds_add_f32 measured 14
     // times slower than that on gfx950).
     static __device__ __forceinline__ void add(LdsF* p, F v)
     {
