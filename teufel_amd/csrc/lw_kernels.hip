@@ -473,8 +473,8 @@ __device__ __forceinline__ void deposit_window(const Window<float>& win, int W, 
 // coalesced vector loads one chunk ahead of use, and the wave reads the
 // current record from LDS.  The wait for a chunk falls once per 16 steps.
 //
-// ScalarPrefetch (mixed kernel, 64-B records): one scalar load per record,
-// requested one step ahead of use.
+// ScalarPrefetch (mixed kernel, 64-B records): the record's four 16-B pieces
+// (blocked layout) read with scalar loads, requested one step ahead of use.
 constexpr int kChunkRecs = 16;
 constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
 
