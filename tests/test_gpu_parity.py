@@ -421,3 +421,7 @@ def test_errors_are_reported(ctx):
         ctx.add_screen(np.array([[0, 0, math.nan]]), np.zeros(1), 1e-15, 10)
     with pytest.raises(teufel_amd.TgError):
         ctx._chk(ctx.lib.tg_zero_screen(ctx.h, 9999))
+    # more points per particle than the kernel's 32-bit record offsets allow
+    # (no particles, so no array is read)
+    with pytest.raises(teufel_amd.TgError, match="points per particle"):
+        ctx._chk(ctx.lib.tg_set_trajectories(ctx.h, 0, 23_000_000, *([None] * 10), None))

@@ -699,6 +699,8 @@ int set_traj_common(tg_context* c, int n_particles, int n_points, const double* 
                     hipMemcpyKind kind)
 {
     if (n_particles < 0 || n_points < 0) return fail(c, TG_ERR_ARG, "negative trajectory size");
+    if ((long long)rec_particle_bytes<RecF64>(n_points) > (long long)INT_MAX)
+        return fail(c, TG_ERR_ARG, "more than 22 million points per particle");
     for (int i = 0; i < kNumRaw; ++i)
         if (!src[i] && (size_t)n_particles * n_points > 0) return fail(c, TG_ERR_ARG, "null trajectory array");
     if (!charge && n_particles > 0) return fail(c, TG_ERR_ARG, "null charge array");
