@@ -270,17 +270,15 @@ struct RcpF32 {
 };
 
 // LDS trace window of one wave: cell (bucket j, component c, lane) at
-// ((j*6)+c)*64 + lane, W buckets, plus for the mixed kernel one per-lane
-// trash bucket (W) that absorbs the absent targets of the branch-free common
-// deposit (both targets may land there; its contents are never read); the
-// FP64 kernel masks its adds instead (trash_buckets).  Every cell has exactly one
-// writer (its own lane) and a wave's LDS accesses execute in order, so plain
-// read-add-write is race-free and deterministic.
+// ((j*6)+c)*64 + lane, W buckets.  A step's absent targets (negative) mask
+// their lanes off the adds, so no cell outside the window is touched.  Every
+// cell has exactly one writer (its own lane) and a wave's LDS accesses
+// execute in order, so plain read-add-write is race-free and deterministic.
 template <class F>
 struct Window {
     using LdsF = __attribute__((address_space(3))) F;
     LdsF* lane_base;  // lds + lane
-    // j in [0, W]: a 24-bit product (one full-rate v_mad_u32_u24 for the address)
+    // j in [0, W): a 24-bit product (one full-rate v_mad_u32_u24 for the address)
     __device__ __forceinline__ LdsF* cell(int j) const { return lane_base + __mul24(j, 6 * kLanes); }
     // *p += v.  FP64 cells add in the LDS unit (ds_add_f64): each lane owns
     // its column, so no two lanes meet at one address and the adds of a lane
@@ -294,10 +292,8 @@ struct Window {
         else
             *p += v;
     }
-    // bucket ja += aa f1 + ba f2 and bucket jb += ab f1 + bb f2 (ja != jb).
-    // FP64: an absent target is negative and its adds are masked off.
-    // FP32: an absent target is the trash bucket (both may be), so the
-    // read-add-write needs no mask.
+    // bucket ja += aa f1 + ba f2 and bucket jb += ab f1 + bb f2 (ja != jb);
+    // a negative (absent) target masks the lane off its adds.
     __device__ __forceinline__ void add2(int ja, F aa, F ba, int jb, F ab, F bb, const F* f1, const F* f2) const
     {
         if constexpr (sizeof(F) == 8) {
@@ -315,17 +311,16 @@ struct Window {
             // cell + (a f1 + b f2): one rounding at the cell's magnitude per
             // step (folding the add into the FMAs would make it two, and
             // the window's FP32 sums measurably worse)
-            LdsF* ca = cell(ja);
-            LdsF* cb = cell(jb);
-            F va[6], vb[6];
+            if (ja >= 0) {
+                LdsF* ca = cell(ja);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) va[c] = ca[c * kLanes];
+                for (int c = 0; c < 6; ++c) ca[c * kLanes] = ca[c * kLanes] + (aa * f1[c] + ba * f2[c]);
+            }
+            if (jb >= 0) {
+                LdsF* cb = cell(jb);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) vb[c] = cb[c * kLanes];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) ca[c * kLanes] = va[c] + (aa * f1[c] + ba * f2[c]);
-#pragma unroll
-            for (int c = 0; c < 6; ++c) cb[c * kLanes] = vb[c] + (ab * f1[c] + bb * f2[c]);
+                for (int c = 0; c < 6; ++c) cb[c * kLanes] = cb[c * kLanes] + (ab * f1[c] + bb * f2[c]);
+            }
         }
     }
 };
@@ -464,7 +459,7 @@ __device__ __forceinline__ void deposit_window(const Window<float>& win, int W, 
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
     const TwoBuckets<float> t =
-        step_common_units<float>(RcpF32{}, valid && !gen, p1.idx, p1.u, p2.idx, p2.u, i2, b0, b1, W, W);
+        step_common_units<float>(RcpF32{}, valid && !gen, p1.idx, p1.u, p2.idx, p2.u, i2, b0, b1, -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
@@ -854,8 +849,7 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
     }
     const Window<F> win{(typename Window<F>::LdsF*)(reinterpret_cast<F*>(lds)) + lane};
     // FP64 wave-uniform walk: two record chunks after the window
-    double* stage =
-        reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)(lds_window + trash_buckets(PREC)) * 6 * kLanes);
+    double* stage = reinterpret_cast<double*>(reinterpret_cast<F*>(lds) + (size_t)lds_window * 6 * kLanes);
     // wave-uniform walk: seed the cursors of a group that does not start at
     // bucket 0 (the per-lane walk searches each lane's start itself)
     const bool seeded = !LANES && it.bin_lo > 0;

@@ -377,8 +377,8 @@ int default_window(const tg_context* c, int n_bins)
 
 size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
 
-// LDS of one wave: (window + 1 trash bucket) x 6 components x 64 lanes, plus
-// the FP64 kernel's record chunks.
+// LDS of one wave: window x 6 components x 64 lanes, plus the FP64
+// wave-uniform walk's record chunks.
 size_t stage_bytes(int prec, bool lanes)
 {
     if (lanes) return 0;  // the per-lane walk reads records straight from global memory
@@ -387,7 +387,7 @@ size_t stage_bytes(int prec, bool lanes)
 
 int max_window_for_lds(const tg_context* c, int prec, bool lanes)
 {
-    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec))) - trash_buckets(prec);
+    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec)));
 }
 
 // tg_tuning.walk: 1 = wave-uniform walk, 2 = per-lane walk; 0 picks the
@@ -484,8 +484,7 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     // launch short; a group seeds its cursors by search instead of walking
     // from step 0.
     long long base_items = total_tiles * n_chunks;
-    const size_t lds_wave = (size_t)(plan.max_window + trash_buckets(prec)) * 6 * kLanes * cell_bytes(prec) +
-                            stage_bytes(prec, plan.lanes);
+    const size_t lds_wave = (size_t)plan.max_window * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
     long long slots = (long long)std::max(c->n_cu, 1) * (long long)std::max<size_t>(1, c->max_lds / lds_wave);
     int groups = (int)std::max<long long>(1, std::min<long long>(64, (16 * slots + base_items - 1) / base_items));
     if (tun && tun->window_groups > 0) groups = tun->window_groups;

@@ -28,12 +28,6 @@ constexpr double kInvSpeedOfLight = 1.0 / kSpeedOfLight;  // RN(1/c), folded by 
 
 constexpr int kLanes = 64;  // one wavefront = one pixel tile
 
-// Trash buckets after a wave's LDS window: the mixed kernel (precision 1)
-// sends the absent targets of its branch-free read-add-write deposit to one
-// per-lane trash bucket; the FP64 kernel (precision 0) masks its LDS atomic
-// adds instead and needs none, which leaves room for more waves per CU.
-constexpr int trash_buckets(int prec) { return prec == 0 ? 0 : 1; }
-
 // Packed trajectory point for the FP64 kernel: 12 doubles = 96 B, read per
 // lane (vector loads) by the per-lane walk, or staged through LDS for the
 // wave-uniform walk.  beta = u/gamma and
