@@ -236,11 +236,13 @@ def main():
             log("autotune fp64: %s" % ctx.autotune(TG_FP64))
         w64, k64, t64 = time_integrate(ctx, TG_FP64, args.fp64_steps, 0)
         # one clean pass of each precision for the error (timed passes accumulate)
+        log("error check: one FP64 pass")
         for i in ids:
             ctx.zero_screen(i)
         ctx.integrate(TG_FP64)
         S64 = [ctx.screen_poynting(i) for i in ids]
         T64 = [ctx.read_traces(i) for i in ids]
+        log("error check: one mixed pass")
         for i in ids:
             ctx.zero_screen(i)
         ctx.integrate(TG_MIXED, tuning)
