@@ -428,7 +428,7 @@ __device__ __forceinline__ int wave_min(int v)
 
 // Deposit of step (p1 -> p2) into the window [b0, b1), FP64: retarded times
 // and bucket edges as the reference forms them.
-__device__ __forceinline__ void deposit_window(const Window<double>& win, int W, bool active, const Point& p1,
+__device__ __forceinline__ void deposit_window(const Window<double>& win, bool active, const Point& p1,
                                                const Point& p2, double t0, double dt, double inv_dt, double tmin,
                                                double tmax, int n_bins, int b0, int b1)
 {
@@ -450,7 +450,7 @@ __device__ __forceinline__ void deposit_window(const Window<double>& win, int W,
 
 // Mixed: the same rule in bucket units (deposit_step_units), the common
 // step branch-free with FP32 weights, the general step under a ballot.
-__device__ __forceinline__ void deposit_window(const Window<float>& win, int W, bool active, const PointM& p1,
+__device__ __forceinline__ void deposit_window(const Window<float>& win, bool active, const PointM& p1,
                                                const PointM& p2, double, double, double, double, double,
                                                int n_bins, int b0, int b1)
 {
@@ -602,7 +602,7 @@ template <int PREC>
 __device__ __forceinline__ int walk_particle(const char* __restrict__ rp, int k,
                                              int NP, bool active, double Px, double Py, double Pz,
                                              double t0, double dt, double inv_dt, double tmin,
-                                             double tmax, int n_bins, int b0, int b1, int W,
+                                             double tmax, int n_bins, int b0, int b1,
                                              const Window<typename RecOf<PREC>::amp>& win, double* stage,
                                              int lane, bool verify, int& nk, int& nmin)
 {
@@ -632,7 +632,7 @@ __device__ __forceinline__ int walk_particle(const char* __restrict__ rp, int k,
             break;
         }
         eval_point<PREC>(src.next(k + 1), Px, Py, Pz, t0, dt, inv_dt, p1);
-        deposit_window(win, W, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        deposit_window(win, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
         if (next_k < 0 && __ballot(active && p1.idx >= b1)) {
             next_k = k;
@@ -695,7 +695,7 @@ template <int PREC>
 __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, int NP,
                                           bool active, double Px, double Py, double Pz, double t0, double dt,
                                           double inv_dt, double tmin, double tmax, int n_bins, int b0, int b1,
-                                          int W, const Window<typename RecOf<PREC>::amp>& win, bool search,
+                                          const Window<typename RecOf<PREC>::amp>& win, bool search,
                                           const unsigned char* __restrict__ lc_in, unsigned char* __restrict__ lc_out,
                                           int& nk, int& nmin)
 {
@@ -757,7 +757,7 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
         __builtin_amdgcn_sched_barrier(0);
         rnext = rbuf.load(kl + 2);
         eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
-        deposit_window(win, W, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        deposit_window(win, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
         if (walking) {
             if (pe.idx >= b1) {
@@ -909,12 +909,12 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                     // item stored them, written when a next window follows
                     unsigned char* lc = lane_cur ? lane_cur + (size_t)(p - it.p0) * kLanes : nullptr;
                     ws = walk_lanes<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin, tmax,
-                                          nb, b0, b1, W, win, first, first ? nullptr : lc,
+                                          nb, b0, b1, win, first, first ? nullptr : lc,
                                           b1 < it.bin_hi ? lc : nullptr, nk, nmin);
                 } else {
                     const bool verify = first && k > 0;
                     ws = walk_particle<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin,
-                                             tmax, nb, b0, b1, W, win, stage, lane, verify, nk, nmin);
+                                             tmax, nb, b0, b1, win, stage, lane, verify, nk, nmin);
                 }
                 n_steps += ws;
                 ++n_walks;
