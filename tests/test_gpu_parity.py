@@ -351,6 +351,50 @@ def test_graph_replay_matches_eager(ctx):
         assert np.array_equal(out, eager), rep
 
 
+def test_graph_replay_after_eager_launch_of_another_plan(ctx):
+    say("hipGraph capture, eager launches of other plans (FP64, a larger screen), replay: bitwise equal to eager")
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=16, n_steps=2000)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 256)
+    ctx.integrate(TG_MIXED)
+    eager = ctx.read_traces(sid)
+    ctx.zero_screen(sid)
+    ctx.integrate(TG_MIXED, tuning={"use_graph": 1})          # capture + launch
+    assert np.array_equal(ctx.read_traces(sid), eager)
+    # eager launches of other plans between the replays: another precision,
+    # then a plan whose work list and scratch are larger (finer chunks)
+    ctx.zero_screen(sid)
+    ctx.integrate(TG_FP64)
+    ref64 = oracle.integrate_screen(tr, obs, t0, dt, 256)
+    assert trace_rel_error(ctx.read_traces(sid), ref64) <= FP64_TOL
+    for tun in ({"use_graph": 1}, {"particles_per_chunk": 1, "bins_per_window": 3}, {"use_graph": 1}):
+        ctx.zero_screen(sid)
+        ctx.integrate(TG_MIXED, tuning=tun)
+        out = ctx.read_traces(sid)
+        if tun.get("use_graph"):
+            assert np.array_equal(out, eager), tun
+        else:
+            assert trace_rel_error(out, eager) <= 1e-5
+
+
+def test_autotune_failed_trial_restores_traces(ctx, monkeypatch):
+    say("tg_autotune whose 3rd trial fails (TG_AUTOTUNE_FAIL_TRIAL): error reported, traces unchanged")
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=16, n_steps=1000)
+    monkeypatch.setenv("TG_AUTOTUNE_FAIL_TRIAL", "3")
+    with teufel_amd.RadiationContext(0) as c2:
+        c2.set_trajectories(tr)
+        sid = c2.add_screen(obs, t0, dt, 256)
+        c2.integrate(TG_FP64)
+        before = c2.read_traces(sid)
+        with pytest.raises(teufel_amd.TgError, match="TG_AUTOTUNE_FAIL_TRIAL"):
+            c2.autotune(TG_MIXED)
+        assert np.array_equal(c2.read_traces(sid), before)
+        # the context still works, untuned
+        c2.integrate(TG_FP64)
+        assert np.allclose(c2.read_traces(sid), 2 * before, rtol=1e-14, atol=0)
+
+
 @pytest.mark.parametrize("prec", [TG_FP64, TG_MIXED])
 def test_autotune_keeps_traces_and_tuned_launch_matches_oracle(ctx, prec):
     say("autotune (window, groups, chunk) on 2 screens, then tuned launch vs oracle, prec=%d" % prec)

@@ -36,10 +36,32 @@ def _dptr(a):
 
 
 def _f64(a, shape=None):
+    """An input array as C-contiguous float64 (a copy where needed)."""
     a = np.ascontiguousarray(a, dtype=np.float64)
     if shape is not None:
         a = a.reshape(shape)
     return a
+
+
+def _check_shape(a, shape, name):
+    if a.shape != tuple(shape):
+        raise ValueError("%s: shape %s, expected %s" % (name, a.shape, tuple(shape)))
+    return a
+
+
+def _out(a, shape, name):
+    """An output buffer handed to the library: it is written in place, so it
+    must already be C-contiguous float64 of exactly the expected shape (a
+    converted copy would silently receive the result instead)."""
+    if not isinstance(a, np.ndarray):
+        raise ValueError("%s: expected a numpy array, got %s" % (name, type(a).__name__))
+    if a.dtype != np.float64:
+        raise ValueError("%s: dtype %s, expected float64" % (name, a.dtype))
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("%s: array is not C-contiguous" % name)
+    if not a.flags["WRITEABLE"]:
+        raise ValueError("%s: array is read-only" % name)
+    return _check_shape(a, shape, name)
 
 
 class RadiationContext:
@@ -85,17 +107,24 @@ class RadiationContext:
     # ---- trajectories ----
     def set_trajectories(self, tr):
         """tr: object with t,x,y,z,ux,uy,uz,ax,ay,az (n_particles, n_points) and charge."""
-        arrs = [_f64(getattr(tr, n)) for n in ("t", "x", "y", "z", "ux", "uy", "uz", "ax", "ay", "az")]
-        n_p, n_pts = arrs[0].shape if arrs[0].ndim == 2 else (1, arrs[0].shape[0])
-        q = _f64(tr.charge).reshape(-1)
+        names = ("t", "x", "y", "z", "ux", "uy", "uz", "ax", "ay", "az")
+        arrs = [_f64(getattr(tr, n)) for n in names]
+        if arrs[0].ndim == 1:
+            arrs = [a.reshape(1, -1) if a.ndim == 1 else a for a in arrs]
+        if arrs[0].ndim != 2:
+            raise ValueError("t: expected (n_particles, n_points), got shape %s" % (arrs[0].shape,))
+        n_p, n_pts = arrs[0].shape
+        for n, a in zip(names, arrs):
+            _check_shape(a, (n_p, n_pts), n)
+        q = _check_shape(_f64(tr.charge).reshape(-1), (n_p,), "charge")
         self._keep = arrs + [q]
         self._chk(self.lib.tg_set_trajectories(self.h, n_p, n_pts, *[_dptr(a) for a in arrs], _dptr(q)))
 
     def track_undulator(self, x0, u0, dt, n_steps, bpeak, lambda_u, n_periods, origin,
                         charge=-1.0, mass=1.0, t_start=0.0):
         x0 = _f64(x0, (-1, 3))
-        u0 = _f64(u0, (-1, 3))
-        o = _f64(origin, (3,))
+        u0 = _check_shape(_f64(u0, (-1, 3)), x0.shape, "u0")
+        o = _check_shape(_f64(origin).reshape(-1), (3,), "origin")
         self._chk(self.lib.tg_track_undulator(self.h, x0.shape[0], _dptr(x0), _dptr(u0), t_start, charge,
                                               mass, dt, n_steps, bpeak, lambda_u, n_periods, _dptr(o)))
 
@@ -116,8 +145,10 @@ class RadiationContext:
 
     # ---- screens ----
     def add_screen(self, obs, t0, dt, n_bins):
-        obs = _f64(obs, (-1, 3))
-        t0 = _f64(t0, (-1,))
+        obs = _f64(obs)
+        if obs.ndim != 2 or obs.shape[1] != 3:
+            raise ValueError("obs: expected (n_pix, 3), got shape %s" % (obs.shape,))
+        t0 = _check_shape(_f64(t0).reshape(-1), (obs.shape[0],), "t0")
         sid = ctypes.c_int()
         self._chk(self.lib.tg_add_screen(self.h, obs.shape[0], _dptr(obs), _dptr(t0), float(dt), int(n_bins),
                                          ctypes.byref(sid)))
@@ -171,6 +202,7 @@ class RadiationContext:
         if out is None:
             out = np.zeros((n_pix, n_bins, 6))
             accumulate = False
+        _out(out, (n_pix, n_bins, 6), "out")
         self._chk(self.lib.tg_read_traces(self.h, sid, _dptr(out), 1 if accumulate else 0))
         return out
 
@@ -187,8 +219,8 @@ class RadiationContext:
         (f[n_bins//2], dens_h[n_pix, n_bins//2], dens_v[n_pix, n_bins//2])."""
         n_pix, n_bins = self.screens[sid]
         nh = n_bins // 2
-        ex = np.ascontiguousarray(ex, dtype=np.float64)
-        ey = np.ascontiguousarray(ey, dtype=np.float64)
+        ex = _check_shape(_f64(ex).reshape(-1), (3,), "ex")
+        ey = _check_shape(_f64(ey).reshape(-1), (3,), "ey")
         f = np.zeros(max(nh, 1))
         dh = np.zeros((n_pix, max(nh, 1)))
         dv = np.zeros((n_pix, max(nh, 1)))
@@ -205,13 +237,21 @@ class RadiationContext:
     def fields_at(self, times, points, p_range=None, out=None, accumulate=False):
         """Field (E, B) at points[i] at times[i] summed over the particles
         (ChargedParticle/Bunch/Beam::RetardedField(t, x)); returns (n, 6)."""
-        points = _f64(points).reshape(-1, 3)
-        times = _f64(np.broadcast_to(np.asarray(times, dtype=np.float64), (points.shape[0],)))
-        n_particles, _ = self.trajectory_shape()
-        p0, p1 = p_range if p_range is not None else (0, n_particles)
+        points = _f64(points)
+        if points.ndim == 1:
+            points = points.reshape(1, -1)
+        if points.ndim != 2 or points.shape[1] != 3:
+            raise ValueError("points: expected (n, 3), got shape %s" % (points.shape,))
+        times = np.asarray(times, dtype=np.float64)
+        if times.ndim == 0:
+            times = np.full(points.shape[0], float(times))
+        times = _check_shape(_f64(times), (points.shape[0],), "times")
         if out is None:
             out = np.zeros((points.shape[0], 6))
             accumulate = False
+        _out(out, (points.shape[0], 6), "out")
+        n_particles, _ = self.trajectory_shape()
+        p0, p1 = p_range if p_range is not None else (0, n_particles)
         self._chk(self.lib.tg_fields_at(self.h, points.shape[0], _dptr(times), _dptr(points), int(p0), int(p1),
                                         _dptr(out), 1 if accumulate else 0))
         return out
@@ -220,9 +260,15 @@ class RadiationContext:
         ts = _f64(ts)
         fields = _f64(fields)
         if ts.ndim == 1:
-            ts, fields = ts[None], fields[None]
+            ts = ts[None]
+        if fields.ndim == 2:
+            fields = fields[None]
+        if ts.ndim != 2:
+            raise ValueError("ts: expected (n_pseudo, n_points), got shape %s" % (ts.shape,))
+        _check_shape(fields, ts.shape + (6,), "fields")
         if trace is None:
             trace = np.zeros((max(int(n_bins), 0), 6))
+        _out(trace, (max(int(n_bins), 0), 6), "trace")
         self._chk(self.lib.tg_deposit_only(self.h, ts.shape[0], ts.shape[1], _dptr(ts), _dptr(fields),
                                            float(t0), float(dt), int(n_bins), _dptr(trace)))
         return trace

@@ -95,6 +95,7 @@ struct tg_context {
     int default_walk = 2;
     bool lane_cursors_on = true;  // TG_LANE_CURSORS=0 turns the per-lane offsets off
     int max_slots = 0;            // TG_SLOTS=n caps the persistent grid at n waves (0: resident waves)
+    int fail_trial = 0;           // TG_AUTOTUNE_FAIL_TRIAL=n fails tg_autotune's n-th trial (error-path tests)
     unsigned long long* d_stats = nullptr;
     size_t cap_stats = 0;
     long long last_steps = 0, last_walks = 0, last_items = 0;
@@ -550,9 +551,13 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     return TG_OK;
 }
 
-// Allocate scratch and upload the plan; fills plan.sdev[].partial.
+// Allocate scratch and upload the plan; fills plan.sdev[].partial.  A
+// captured graph refers to the scratch buffers and the uploaded work list, so
+// any upload (which may reallocate them or overwrite the list with another
+// plan's) invalidates it.
 int upload_plan(tg_context* c, LaunchPlan& plan)
 {
+    free_graph(c);
     int rc;
     if ((rc = ensure_cap(c, c->d_partial, c->cap_partial, plan.partial_doubles))) return rc;
     if ((rc = ensure_cap(c, c->d_slots, c->cap_slots, (size_t)plan.slots * plan.slot_bytes))) return rc;
@@ -629,11 +634,16 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
     if ((rc = ensure_records(c, prec))) return rc;
     for (int id : ids)
         if ((rc = ensure_tiles(c, c->screens[id]))) return rc;
-    tg_tuning eff = {};
-    if (tun)
-        eff = *tun;
-    else if (c->has_tuned[prec])
-        eff = c->tuned[prec];
+    // the autotuned shape, with every non-zero field of the caller's tuning
+    // taking precedence
+    tg_tuning eff = c->has_tuned[prec] ? c->tuned[prec] : tg_tuning{};
+    if (tun) {
+        if (tun->bins_per_window > 0) eff.bins_per_window = tun->bins_per_window;
+        if (tun->particles_per_chunk > 0) eff.particles_per_chunk = tun->particles_per_chunk;
+        if (tun->window_groups > 0) eff.window_groups = tun->window_groups;
+        if (tun->walk > 0) eff.walk = tun->walk;
+        eff.use_graph = tun->use_graph;
+    }
     bool use_graph = eff.use_graph != 0;
     char keybuf[256];
     snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk,
@@ -650,8 +660,7 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         return TG_OK;
     }
     LaunchPlan plan;
-    if ((rc = make_plan(c, prec, ids, p0, p1, tun ? tun : (c->has_tuned[prec] ? &c->tuned[prec] : nullptr), plan)))
-        return rc;
+    if ((rc = make_plan(c, prec, ids, p0, p1, &eff, plan))) return rc;
     if ((rc = upload_plan(c, plan))) return rc;
     if (use_graph) {
         free_graph(c);
@@ -754,6 +763,8 @@ int tg_create(int device, tg_context** out)
     c->lane_cursors_on = !(lcv && lcv[0] == '0');
     const char* sl = getenv("TG_SLOTS");
     c->max_slots = sl ? std::max(0, atoi(sl)) : 0;
+    const char* ft = getenv("TG_AUTOTUNE_FAIL_TRIAL");
+    c->fail_trial = ft ? std::max(0, atoi(ft)) : 0;
     const void* kernels[] = {(const void*)radiation_kernel<0, false>, (const void*)radiation_kernel<1, false>,
                              (const void*)radiation_kernel<0, true>, (const void*)radiation_kernel<1, true>};
     for (const void* k : kernels)
@@ -1237,70 +1248,90 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     float best_ms = 1e30f;
     int best_w = default_window(c, max_bins);
     LaunchPlan best_full;
-    for (int w : cands) {
-        if (w > wmax || w > max_bins) continue;
-        tg_tuning t = {};
-        t.bins_per_window = w;
-        LaunchPlan full;
-        if ((rc = make_plan(c, precision, ids, 0, c->n_particles, &t, full))) return rc;
-        t.particles_per_chunk = sample_chunk(full.n_chunks);
-        t.window_groups = full.groups;
-        if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
-        if (log)
-            std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms (%d particles)\n", w, full.groups,
-                         full.chunk, c->kernel_ms, sample);
-        if (c->kernel_ms < best_ms) {
-            best_ms = c->kernel_ms;
-            best_w = w;
-            best_full = full;
-        }
-    }
-    // Second stage, at the chosen window: the number of window groups (item
-    // granularity against the search each group costs) and the particle
-    // chunk (items per tile against partial traces to reduce), each halved
-    // and doubled around the default.  A shape replaces the default only
-    // when it is more than 2 % faster, so timing noise does not move it.
-    if (best_full.chunk > 0) {
-        const int g0 = best_full.groups, ch0 = best_full.chunk;
-        int best_g = g0, best_ch = ch0;
-        const float base_ms = best_ms;
-        float shape_ms = base_ms;
-        size_t per_chunk = 0;
-        for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
-        const int shapes[4][2] = {{g0 / 2, ch0}, {g0 * 2, ch0}, {g0, ch0 * 2}, {g0, ch0 / 2}};
-        for (const auto& sh : shapes) {
-            const int g = sh[0], ch = sh[1];
-            if (g < 1 || g > 64 || ch < 1 || ch > c->n_particles) continue;
-            const size_t n_ch = ((size_t)c->n_particles + ch - 1) / ch;
-            if (n_ch * per_chunk * sizeof(double) > ((size_t)16 << 30)) continue;
+    int n_trials = 0;
+    // after a trial launch has added the sample into the traces
+    auto injected = [&]() {
+        return ++n_trials == c->fail_trial
+                   ? fail(c, TG_ERR_STATE, "tg_autotune: trial " + std::to_string(n_trials) +
+                                               " failed (TG_AUTOTUNE_FAIL_TRIAL)")
+                   : TG_OK;
+    };
+    auto trials = [&]() -> int {
+        for (int w : cands) {
+            if (w > wmax || w > max_bins) continue;
             tg_tuning t = {};
-            t.bins_per_window = best_w;
-            t.particles_per_chunk = sample_chunk((long long)n_ch);
-            t.window_groups = g;
-            if ((rc = integrate_impl(c, precision, &t, -1, 0, sample))) return rc;
+            t.bins_per_window = w;
+            LaunchPlan full;
+            if ((rc = make_plan(c, precision, ids, 0, c->n_particles, &t, full))) return rc;
+            t.particles_per_chunk = sample_chunk(full.n_chunks);
+            t.window_groups = full.groups;
+            if ((rc = integrate_impl(c, precision, &t, -1, 0, sample)) || (rc = injected())) return rc;
             if (log)
-                std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms\n", best_w, g, ch, c->kernel_ms);
-            if (c->kernel_ms < shape_ms) {
-                shape_ms = c->kernel_ms;
-                best_g = g;
-                best_ch = ch;
+                std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms (%d particles)\n", w, full.groups,
+                             full.chunk, c->kernel_ms, sample);
+            if (c->kernel_ms < best_ms) {
+                best_ms = c->kernel_ms;
+                best_w = w;
+                best_full = full;
             }
         }
-        if (shape_ms < 0.98f * base_ms && (best_g != g0 || best_ch != ch0)) {
-            best.window_groups = best_g;
-            best.particles_per_chunk = best_ch;
+        // Second stage, at the chosen window: the number of window groups (item
+        // granularity against the search each group costs) and the particle
+        // chunk (items per tile against partial traces to reduce), each halved
+        // and doubled around the default.  A shape replaces the default only
+        // when it is more than 2 % faster, so timing noise does not move it.
+        if (best_full.chunk > 0) {
+            const int g0 = best_full.groups, ch0 = best_full.chunk;
+            int best_g = g0, best_ch = ch0;
+            const float base_ms = best_ms;
+            float shape_ms = base_ms;
+            size_t per_chunk = 0;
+            for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
+            const int shapes[4][2] = {{g0 / 2, ch0}, {g0 * 2, ch0}, {g0, ch0 * 2}, {g0, ch0 / 2}};
+            for (const auto& sh : shapes) {
+                const int g = sh[0], ch = sh[1];
+                if (g < 1 || g > 64 || ch < 1 || ch > c->n_particles) continue;
+                const size_t n_ch = ((size_t)c->n_particles + ch - 1) / ch;
+                if (n_ch * per_chunk * sizeof(double) > ((size_t)16 << 30)) continue;
+                tg_tuning t = {};
+                t.bins_per_window = best_w;
+                t.particles_per_chunk = sample_chunk((long long)n_ch);
+                t.window_groups = g;
+                if ((rc = integrate_impl(c, precision, &t, -1, 0, sample)) || (rc = injected())) return rc;
+                if (log)
+                    std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms\n", best_w, g, ch, c->kernel_ms);
+                if (c->kernel_ms < shape_ms) {
+                    shape_ms = c->kernel_ms;
+                    best_g = g;
+                    best_ch = ch;
+                }
+            }
+            if (shape_ms < 0.98f * base_ms && (best_g != g0 || best_ch != ch0)) {
+                best.window_groups = best_g;
+                best.particles_per_chunk = best_ch;
+            }
         }
-    }
+        return TG_OK;
+    };
+    // the trials add the sample's fields into the screens: restore every
+    // screen's traces whether or not the trials succeeded
+    const int trial_rc = trials();
+    int restore_rc = TG_OK;
     for (size_t id = 0; id < c->screens.size(); ++id) {
-        ScreenHost& s = c->screens[id];
-        s.window[precision] = best_w;
         if (saved[id].empty()) continue;
-        TG_HIP(c, hipMemcpy(s.d_traces, saved[id].data(), saved[id].size() * sizeof(double), hipMemcpyHostToDevice));
+        ScreenHost& s = c->screens[id];
+        const hipError_t e = hipMemcpy(s.d_traces, saved[id].data(), saved[id].size() * sizeof(double),
+                                       hipMemcpyHostToDevice);
+        if (e != hipSuccess && restore_rc == TG_OK)
+            restore_rc = fail(c, TG_ERR_HIP, std::string("tg_autotune: restoring traces: ") + hipGetErrorString(e));
     }
+    free_graph(c);
+    if (trial_rc) return trial_rc;
+    if (restore_rc) return restore_rc;
+    for (ScreenHost& s : c->screens) s.window[precision] = best_w;
     best.bins_per_window = 0;  // per-screen windows stored on the screens
     c->tuned[precision] = best;
     c->has_tuned[precision] = true;
-    free_graph(c);
     if (out) {
         *out = best;
         int w = 0;
