@@ -25,11 +25,19 @@
 //       <trajectory>, written by tests/test_host_api.py), integrate, print
 //       totalEnergy(); the Python test checks 0.463 +- 0.020 uJ (the reference
 //       test's own bound) and the traces against the oracle
-//   host_api_test screens <input> <mesh.dump> <screen.dump> [fp64|mixed]
+//   host_api_test screens <input> <mesh.dump> <screen.h5> [fp64|mixed]
 //       trajectories and a mesh definition from <input> (written by
 //       tests/test_host_api.py), integrated by MeshedScreen<Beam> (two
-//       bunches) and ScreenObserver<Bunch>; both dumps are compared against
-//       the oracle by the Python test.  Prints the mesh's totalEnergy().
+//       bunches) and ScreenObserver<Bunch> (whose output is the HDF5 file of
+//       src/screen.cpp:297-558); both are compared against the oracle by the
+//       Python test.  Prints the mesh's totalEnergy().
+//   host_api_test screenh5 <out.h5>
+//       a ScreenObserver<Bunch> with known field values written by
+//       generateOutput() and read back with readScreenFile: geometry, cell
+//       positions, sampling and every value must survive (no GPU needed)
+//   host_api_test screenread <screen.h5> <out.dump>
+//       reads a ScreenObserver HDF5 file and writes its cells and traces as a
+//       plain dump (positions, cell t0, samples) for the Python tests
 //
 // Exit status: number of failed checks (0 = pass), 100+ on usage/IO errors.
 #include <algorithm>
@@ -394,6 +402,62 @@ int run_diffraction(const char* h5, const char* traj, const char* out)
     return std::fabs(EuJ - 0.463) > 0.020;
 }
 
+// ScreenObserver HDF5 file written by generateOutput and read back
+int run_screenh5(const char* out)
+{
+    const Vector origin(0.01, -0.02, 10.0), dx(0.004, 0.0, 0.0005), dy(0.0, 0.005, 0.0);
+    const unsigned nx = 9, ny = 7, nots = 16;
+    const double t0 = 3.3e-8, dt = 2.5e-15, dtx = 1e-15, dty = -2e-15;
+    ScreenObserver<Bunch> scr(out, origin, dx, dy, nx, ny, t0, dt, dtx, dty, nots);
+    auto value = [&](unsigned ix, unsigned iy, unsigned it, int c) { return ix * 1000.0 + iy * 10.0 + it * 1e-3 + c * 1e-6; };
+    for (unsigned ix = 0; ix < nx; ix++)
+        for (unsigned iy = 0; iy < ny; iy++)
+            for (unsigned it = 0; it < nots; it++)
+                scr.setField(ix, iy, it,
+                             ElMagField(Vector(value(ix, iy, it, 0), value(ix, iy, it, 1), value(ix, iy, it, 2)),
+                                        Vector(value(ix, iy, it, 3), value(ix, iy, it, 4), value(ix, iy, it, 5))));
+    scr.generateOutput();
+    const ScreenFile f = readScreenFile(out);
+    int errors = 0;
+    auto same = [](const Vector& a, const Vector& b) { return a.x == b.x && a.y == b.y && a.z == b.z; };
+    if (f.Nx != nx || f.Ny != ny || f.NOTS != nots) errors++;
+    if (!same(f.origin, origin) || !same(f.dX, dx) || !same(f.dY, dy) || !same(f.normal, scr.getNormal())) errors++;
+    if (f.t0 != t0 || f.dt != dt || f.dtx != dtx || f.dty != dty) errors++;
+    // normal = -(dX x dY) / |dX x dY| (src/screen.cpp:51-52)
+    const Vector n = cross(dx, dy) * -1.0;
+    if (std::fabs(f.normal.x - n.x / n.norm()) > 1e-16 || std::fabs(f.normal.z - n.z / n.norm()) > 1e-16) errors++;
+    for (unsigned ix = 0; ix < nx && errors == 0; ix++)
+        for (unsigned iy = 0; iy < ny; iy++) {
+            if (!same(f.position[ix * ny + iy], scr.CellPosition(ix, iy))) errors++;
+            for (unsigned it = 0; it < nots; it++)
+                for (int c = 0; c < 6; c++)
+                    if (f.field[((size_t)(ix * ny + iy) * nots + it) * 6 + c] != value(ix, iy, it, c)) errors++;
+        }
+    std::printf("screenh5: %u x %u cells x %u samples written and read back, errors %d\n", nx, ny, nots, errors);
+    return errors;
+}
+
+// a ScreenObserver file as a plain dump: positions, per-cell t0
+// (src/screen.cpp:106-110), samples
+int run_screenread(const char* in, const char* out)
+{
+    const ScreenFile f = readScreenFile(in);
+    std::vector<Vector> pts;
+    std::vector<FieldTrace*> tr;
+    for (unsigned ix = 0; ix < f.Nx; ix++)
+        for (unsigned iy = 0; iy < f.Ny; iy++) {
+            const double t0 = f.t0 + f.dtx * (double)((int)ix - (int)f.Nx / 2) + f.dty * (double)((int)iy - (int)f.Ny / 2);
+            FieldTrace* t = new FieldTrace(t0, f.dt, f.NOTS);
+            std::memcpy(t->data(), &f.field[(size_t)(ix * f.Ny + iy) * f.NOTS * 6], sizeof(double) * 6 * f.NOTS);
+            pts.push_back(f.position[ix * f.Ny + iy]);
+            tr.push_back(t);
+        }
+    writeDump(out, pts, tr);
+    for (FieldTrace* t : tr) delete t;
+    std::printf("screenread: %u x %u cells x %u samples\n", f.Nx, f.Ny, f.NOTS);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv)
@@ -415,6 +479,10 @@ int main(int argc, char** argv)
             return run_h5info(argv[2]);
         } else if (mode == "h5write" && argc >= 4) {
             return run_h5write(argv[2], argv[3]);
+        } else if (mode == "screenh5" && argc >= 3) {
+            return run_screenh5(argv[2]);
+        } else if (mode == "screenread" && argc >= 4) {
+            return run_screenread(argv[2], argv[3]);
         } else if (mode == "diffraction" && argc >= 5) {
             if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
             errors = run_diffraction(argv[2], argv[3], argv[4]);

@@ -133,6 +133,50 @@ def test_screen_file_writer(tmp_path):
     assert np.array_equal(_h5dump_values(out, "/ObservationTime"), _h5dump_values(H5, "/ObservationTime"))
 
 
+def test_screen_observer_file(tmp_path):
+    """ScreenObserver::generateOutput writes the HDF5 file of
+    src/screen.cpp:297-558 (datasets Screen, ObservationPosition, ElMagField
+    with their attributes, the layout scripts/teufel.py:23-75 reads); it reads
+    back identically, and the HDF5 library's h5dump reads the same structure,
+    attributes and values."""
+    _program()
+    out = str(tmp_path / "screen.h5")
+    r = subprocess.run([BIN, "screenh5", out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    if not _h5dump():
+        pytest.skip("h5dump not available for the independent check")
+    hdr = subprocess.run([_h5dump(), "-H", out], capture_output=True, text=True, timeout=60, cwd="/tmp")
+    assert hdr.returncode == 0, hdr.stderr
+    for name, shape in (("ElMagField", "( 9, 7, 16, 6 )"), ("ObservationPosition", "( 9, 7, 3 )"),
+                        ("Screen", "( 4, 3 )")):
+        block = hdr.stdout.split('DATASET "%s"' % name)[1].split("DATASET")[0]
+        assert shape in block and "H5T_IEEE_F64LE" in block
+    for attr, want in (("Nx", 9), ("Ny", 7), ("NOTS", 16), ("t0", 3.3e-8), ("dt", 2.5e-15), ("dtx", 1e-15),
+                       ("dty", -2e-15)):
+        assert _h5dump_values(out, "/Screen/" + attr, "-a").tolist() == [want], attr
+    assert _h5dump_values(out, "/ElMagField/NOTS", "-a").tolist() == [16]
+    assert _h5dump_values(out, "/ElMagField/dt", "-a").tolist() == [2.5e-15]
+    assert _h5dump_values(out, "/ObservationPosition/Nx", "-a").tolist() == [9]
+    g = _h5dump_values(out, "/Screen").reshape(4, 3)
+    dx, dy = np.array([0.004, 0.0, 0.0005]), np.array([0.0, 0.005, 0.0])
+    n = -np.cross(dx, dy)
+    assert np.array_equal(g[0], [0.01, -0.02, 10.0]) and np.array_equal(g[2], dx) and np.array_equal(g[3], dy)
+    assert np.allclose(g[1], n / np.linalg.norm(n), rtol=0, atol=1e-16)
+    # cell (ix=4, iy=2): origin + dX (4 - 9//2) + dY (2 - 7//2)
+    p = _h5dump_values(out, "/ObservationPosition", "-d", ["-s", "4,2,0", "-c", "1,1,3"])
+    assert np.allclose(p, [0.01, -0.025, 10.0], rtol=0, atol=1e-15)
+    f = _h5dump_values(out, "/ElMagField", "-d", ["-s", "8,6,15,0", "-c", "1,1,1,6"])
+    assert np.array_equal(f, 8000.0 + 60.0 + 15e-3 + 1e-6 * np.arange(6))
+
+
+def test_screen_file_rejects_malformed(tmp_path):
+    """The ScreenObserver file reader raises IOexception (exit status 110 of
+    the test program) for a MeshedScreen file, which lacks the Screen dataset."""
+    _program()
+    r = subprocess.run([BIN, "screenread", H5, str(tmp_path / "x.dump")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 110 and "no dataset Screen" in r.stdout, r.stdout + r.stderr
+
+
 def diffraction_trajectory():
     """tests/teufel.DiffractionScreen.cpp:73-103: 100 pC at gamma 20 on a
     straight line from z = -10 m to +10 m (an empty lattice), passing z = 0 at
@@ -253,10 +297,13 @@ def test_host_screens_vs_oracle(tmp_path, precision):
     scr = dict(origin=[0.0, 0.0, 10.0], dx=[0.004, 0.0, 0.0], dy=[0.0, 0.005, 0.0], nx=9, ny=7,
                t0=float(t0[axis]), dt=dt, dtx=1e-15, dty=-2e-15, nots=nt)
     inp = str(tmp_path / "in.bin")
-    mesh_dump, scr_dump = str(tmp_path / "mesh.dump"), str(tmp_path / "screen.dump")
+    mesh_dump, scr_h5, scr_dump = str(tmp_path / "mesh.dump"), str(tmp_path / "screen.h5"), str(tmp_path / "screen.dump")
     write_input(inp, tr, obs, t0, dt, nt, area, normal, scr)
-    r = run("screens", inp, mesh_dump, scr_dump, precision)
+    r = run("screens", inp, mesh_dump, scr_h5, precision)
     assert r.returncode == 0
+    # the ScreenObserver output is its HDF5 file; read it back through the
+    # file reader into a dump
+    assert run("screenread", scr_h5, scr_dump).returncode == 0
 
     pts, pt0, pdt, mesh = read_dump(mesh_dump)
     assert np.array_equal(pts, obs) and np.array_equal(pt0, t0) and pdt == dt

@@ -110,6 +110,7 @@ void H5LiteFile::read_group(uint64_t btree, uint64_t heap)
                 nodes.push_back(u(q + 8, 8));  // key (heap offset), then child: a node or a symbol node
                 q += 16;
             }
+            if (nodes.size() > 100000) fail("group B-tree too large or cyclic");
         } else if (std::memcmp(&buf[node], "SNOD", 4) == 0) {
             const uint64_t count = u(node + 6, 2);
             for (uint64_t i = 0; i < count; ++i) {
@@ -220,8 +221,14 @@ H5LiteFile::Dataset H5LiteFile::read_object(uint64_t header) const
                 std::vector<uint64_t> adims;
                 read_space(d, adims);
                 d += ver == 1 ? pad8(slen) : slen;
+                bool overflow = false;
                 a.count = 1;
-                for (uint64_t x : adims) a.count *= x;
+                for (uint64_t x : adims) {
+                    if (x != 0 && a.count > ~0ull / x) overflow = true;
+                    a.count *= x;
+                }
+                if (overflow || (a.type.size != 0 && a.count > ~0ull / a.type.size))
+                    fail("attribute " + name + ": dataspace too large");
                 const uint64_t nbytes = a.count * a.type.size;
                 need(d, nbytes);
                 a.data.assign(buf.begin() + (long)d, buf.begin() + (long)(d + nbytes));
@@ -364,7 +371,8 @@ double H5LiteFile::attr_double(const std::string& dataset, const std::string& na
     const Dataset& ds = get(dataset);
     auto it = ds.attrs.find(name);
     if (it == ds.attrs.end()) fail(dataset + ": no attribute " + name);
-    if (!numeric(it->second.type) || it->second.count < 1) fail(dataset + "/" + name + ": not numeric");
+    if (!numeric(it->second.type) || it->second.count < 1 || it->second.data.size() < it->second.type.size)
+        fail(dataset + "/" + name + ": not numeric");
     return element(it->second.type, it->second.data.data());
 }
 
@@ -373,7 +381,8 @@ int64_t H5LiteFile::attr_int(const std::string& dataset, const std::string& name
     const Dataset& ds = get(dataset);
     auto it = ds.attrs.find(name);
     if (it == ds.attrs.end()) fail(dataset + ": no attribute " + name);
-    if (!numeric(it->second.type) || it->second.count < 1) fail(dataset + "/" + name + ": not numeric");
+    if (!numeric(it->second.type) || it->second.count < 1 || it->second.data.size() < it->second.type.size)
+        fail(dataset + "/" + name + ": not numeric");
     return element_int(it->second.type, it->second.data.data());
 }
 

@@ -29,6 +29,38 @@ void writeDump(const std::string& filename, const std::vector<Vector>& points, c
     if (!ok) throw IOexception("writeDump - write error on " + filename);
 }
 
+ScreenFile readScreenFile(const std::string& filename)
+{
+    H5LiteFile f(filename);
+    const char* who = "readScreenFile - ";
+    for (const char* d : {"Screen", "ObservationPosition", "ElMagField"})
+        if (!f.exists(d)) throw IOexception(std::string(who) + filename + " has no dataset " + d);
+    ScreenFile s;
+    const int64_t nx = f.attr_int("Screen", "Nx"), ny = f.attr_int("Screen", "Ny"), nots = f.attr_int("Screen", "NOTS");
+    if (nx < 0 || ny < 0 || nots < 0 || nx > (1 << 20) || ny > (1 << 20) || nots > (1 << 30))
+        throw IOexception(std::string(who) + "implausible screen size in " + filename);
+    s.Nx = (unsigned)nx;
+    s.Ny = (unsigned)ny;
+    s.NOTS = (unsigned)nots;
+    if (f.dims("Screen") != std::vector<uint64_t>{4, 3} ||
+        f.dims("ObservationPosition") != std::vector<uint64_t>{s.Nx, s.Ny, 3} ||
+        f.dims("ElMagField") != std::vector<uint64_t>{s.Nx, s.Ny, s.NOTS, 6})
+        throw IOexception(std::string(who) + "dataset shapes do not match Nx, Ny, NOTS in " + filename);
+    const std::vector<double> g = f.read_double("Screen");
+    s.origin = Vector(g[0], g[1], g[2]);
+    s.normal = Vector(g[3], g[4], g[5]);
+    s.dX = Vector(g[6], g[7], g[8]);
+    s.dY = Vector(g[9], g[10], g[11]);
+    s.t0 = f.attr_double("Screen", "t0");
+    s.dt = f.attr_double("Screen", "dt");
+    s.dtx = f.attr_double("Screen", "dtx");
+    s.dty = f.attr_double("Screen", "dty");
+    const std::vector<double> p = f.read_double("ObservationPosition");
+    for (size_t i = 0; i + 2 < p.size(); i += 3) s.position.emplace_back(p[i], p[i + 1], p[i + 2]);
+    s.field = f.read_double("ElMagField");
+    return s;
+}
+
 namespace {
 double* traces_to_buffer(const std::vector<FieldTrace*>& tr, std::size_t n)
 {
@@ -102,6 +134,14 @@ MeshedScreen<SourceT>::MeshedScreen(const std::string& filename) : Np(0), Nt(0),
         dt = f.attr_double("ObservationTime", "dt");
         t0 = f.read_double("ObservationTime");
         if (t0.size() != (size_t)Np) throw IOexception("MeshedScreen - ObservationTime size mismatch.");
+        // the traces are allocated from Np and the Nt attribute: a file
+        // declaring more than 64 GiB of them is taken as corrupt rather than
+        // exhausting the host's memory
+        if ((double)Np * (double)Nt * 48.0 > 68719476736.0)
+            throw IOexception("MeshedScreen - the file declares " + std::to_string((double)Np * Nt * 48.0 / 1e9) +
+                              " GB of traces (Np x Nt), more than 64 GiB.");
+        if (f.exists("ElMagField") && f.dims("ElMagField") != std::vector<uint64_t>{(uint64_t)Np, (uint64_t)Nt, 6})
+            throw IOexception("MeshedScreen - ElMagField size mismatch.");
     }
     for (int i = 0; i < Np; i++) A.push_back(new FieldTrace(t0[i], dt, (std::size_t)Nt));
     if (f.exists("ElMagField")) {
@@ -252,6 +292,8 @@ ScreenObserver<SourceT>::ScreenObserver(std::string filename, Vector position, V
     : FileName(filename), Origin(position), dX(dx), dY(dy), Nx(nx), Ny(ny), NOTS(nots), t0_obs(t0), dt_obs(dt),
       dtx_obs(dtx), dty_obs(dty)
 {
+    normal = cross(dX, dY) * -1.0;
+    normal.normalize();
     for (unsigned int ix = 0; ix < Nx; ix++)
         for (unsigned int iy = 0; iy < Ny; iy++) Traces.push_back(new FieldTrace(CellTimeZero(ix, iy), dt_obs, NOTS));
 }
@@ -310,12 +352,37 @@ void ScreenObserver<SourceT>::fromBuffer(double* buffer, std::size_t size)
 }
 
 template <typename SourceT>
-void ScreenObserver<SourceT>::generateOutput()
+void ScreenObserver<SourceT>::WriteTimeDomainFieldHDF5()
 {
-    std::vector<Vector> pts;
+    H5LiteWriter w;
+    const int32_t nx = (int32_t)Nx, ny = (int32_t)Ny, nots = (int32_t)NOTS;
+    std::vector<double> pos;
+    pos.reserve((size_t)Nx * Ny * 3);
     for (unsigned int ix = 0; ix < Nx; ix++)
-        for (unsigned int iy = 0; iy < Ny; iy++) pts.push_back(CellPosition(ix, iy));
-    writeDump(FileName, pts, Traces);
+        for (unsigned int iy = 0; iy < Ny; iy++) {
+            const Vector p = CellPosition(ix, iy);
+            pos.insert(pos.end(), {p.x, p.y, p.z});
+        }
+    w.add_double("ObservationPosition", {Nx, Ny, 3}, pos);
+    w.attr_int32("ObservationPosition", "Nx", nx);
+    w.attr_int32("ObservationPosition", "Ny", ny);
+    w.add_double("Screen", {4, 3},
+                 {Origin.x, Origin.y, Origin.z, normal.x, normal.y, normal.z, dX.x, dX.y, dX.z, dY.x, dY.y, dY.z});
+    w.attr_int32("Screen", "Nx", nx);
+    w.attr_int32("Screen", "Ny", ny);
+    w.attr_int32("Screen", "NOTS", nots);
+    w.attr_double("Screen", "t0", t0_obs);
+    w.attr_double("Screen", "dt", dt_obs);
+    w.attr_double("Screen", "dtx", dtx_obs);
+    w.attr_double("Screen", "dty", dty_obs);
+    std::vector<double> fld;
+    fld.reserve(getBufferSize());
+    for (const FieldTrace* t : Traces) fld.insert(fld.end(), t->data(), t->data() + 6 * (size_t)NOTS);
+    w.add_double("ElMagField", {Nx, Ny, NOTS, 6}, fld);
+    w.attr_double("ElMagField", "t0", t0_obs);
+    w.attr_double("ElMagField", "dt", dt_obs);
+    w.attr_int32("ElMagField", "NOTS", nots);
+    w.write(FileName);
 }
 
 // ---- PointObserver ---------------------------------------------------------

@@ -4,8 +4,10 @@
 // integrate() hands all pixels of the observer to the GPU in one batched
 // launch instead of the reference's OpenMP pixel loop (src/mesh.cpp:288-342,
 // src/screen.cpp:112-180, src/point_observer.cpp:62-72).  SourceT is Beam or
-// Bunch.  generateOutput() writes a plain binary dump (documented in
-// writeDump); the reference's HDF5/SDDS writers are not part of this library.
+// Bunch.  generateOutput() writes the reference's HDF5 layouts for
+// MeshedScreen (read from a file) and ScreenObserver; PointObserver (SDDS in
+// the reference) and MeshedScreen built from points write a plain binary dump
+// (documented in writeDump).
 #pragma once
 
 #include <ostream>
@@ -85,6 +87,22 @@ private:
     std::vector<int> triangles;              // MeshTriangles, 3 per cell
 };
 
+// The contents of a ScreenObserver HDF5 file (the layout of
+// ScreenObserver::WriteTimeDomainFieldHDF5, src/screen.cpp:297-558, as
+// scripts/teufel.py:23-75 reads it): dataset Screen [4][3] = origin, normal,
+// dX, dY with attributes Nx, Ny, NOTS, t0, dt, dtx, dty; ObservationPosition
+// [Nx][Ny][3] (attributes Nx, Ny); ElMagField [Nx][Ny][NOTS][6] (attributes
+// t0, dt, NOTS).
+struct ScreenFile {
+    unsigned int Nx = 0, Ny = 0, NOTS = 0;
+    Vector origin, normal, dX, dY;
+    double t0 = 0.0, dt = 0.0, dtx = 0.0, dty = 0.0;
+    std::vector<Vector> position;   // [ix * Ny + iy]
+    std::vector<double> field;      // [ix][iy][it][6]
+};
+// Read such a file; IOexception for a file that does not hold that layout.
+ScreenFile readScreenFile(const std::string& filename);
+
 template <typename SourceT>
 class ScreenObserver : public Observer<SourceT> {
 public:
@@ -99,15 +117,20 @@ public:
     ElMagField getField(unsigned int ix, unsigned int iy, unsigned int it);
     void setField(unsigned int ix, unsigned int iy, unsigned int it, ElMagField field);
 
+    Vector getNormal() const { return normal; }
+
     void integrate() override;
     std::size_t getBufferSize() override { return (std::size_t)Nx * Ny * NOTS * 6; }
     double* getBuffer() override;
     void fromBuffer(double* buffer, std::size_t size) override;
-    void generateOutput() override;
+    // the HDF5 file of src/screen.cpp:297-558 (layout: ScreenFile)
+    void WriteTimeDomainFieldHDF5();
+    void generateOutput() override { WriteTimeDomainFieldHDF5(); }
 
 private:
     std::string FileName;
     Vector Origin, dX, dY;
+    Vector normal;   // -(dX x dY), normalised (src/screen.cpp:51-52)
     unsigned int Nx, Ny, NOTS;
     double t0_obs, dt_obs, dtx_obs, dty_obs;
     std::vector<FieldTrace*> Traces;   // [ix * Ny + iy]
