@@ -60,6 +60,9 @@ typedef struct {
     int window_groups;       /* windows of a (tile, chunk) split into this many work items */
     int walk;                /* 1: the wave walks each particle's steps together (lanes share a
                                 step); 2: every lane (pixel) walks its own steps; 0: default (2) */
+    int spill;               /* per-lane walk: 1 = each window holds one more bucket that takes the
+                                steps crossing into the next window whole, 2 = without it (such steps
+                                are walked again in the next window); 0: autotuned or default (1) */
 } tg_tuning;
 
 /* Context on one device; one trace store and one trajectory set per context.

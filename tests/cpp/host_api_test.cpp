@@ -31,6 +31,12 @@
 //       bunches) and ScreenObserver<Bunch> (whose output is the HDF5 file of
 //       src/screen.cpp:297-558); both are compared against the oracle by the
 //       Python test.  Prints the mesh's totalEnergy().
+//   host_api_test perpixel <input> <threads> [fp64|mixed]
+//       the reference's per-pixel loop: <threads> threads call
+//       Bunch::integrateFieldTrace(point, trace) for their pixels of a
+//       MeshedScreen<Bunch> at once; the result must equal one
+//       MeshedScreen::integrate() of the same screen (the adapter combines
+//       the concurrent calls into batched launches); prints both times
 //   host_api_test screenh5 <out.h5>
 //       a ScreenObserver<Bunch> with known field values written by
 //       generateOutput() and read back with readScreenFile: geometry, cell
@@ -41,6 +47,7 @@
 //
 // Exit status: number of failed checks (0 = pass), 100+ on usage/IO errors.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -48,6 +55,7 @@
 #include <iostream>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bunch.h"
@@ -204,7 +212,16 @@ struct Reader {
     }
 };
 
-int run_screens(const char* input, const char* mesh_out, const char* screen_out)
+// the input of the screens and perpixel modes (written by tests/test_host_api.py)
+struct Input {
+    int32_t np = 0, npts = 0, npix = 0, nt = 0, nx = 0, ny = 0, nots = 0;
+    std::vector<double> a, q, t0, area;
+    double dt = 0, st0 = 0, sdt = 0, dtx = 0, dty = 0;
+    std::vector<Vector> pts, nrm;
+    Vector origin, dx, dy;
+};
+
+int read_input(const char* input, Input& in)
 {
     FILE* f = std::fopen(input, "rb");
     if (!f) {
@@ -212,44 +229,129 @@ int run_screens(const char* input, const char* mesh_out, const char* screen_out)
         return 101;
     }
     Reader r{f};
-    int32_t np = r.one<int32_t>(), npts = r.one<int32_t>();
-    if (!r.ok || np < 1 || npts < 1) return 102;
-    size_t n = (size_t)np * npts;
-    std::vector<double> a(10 * n), q(np);
-    r.get(a.data(), a.size());
-    r.get(q.data(), q.size());
-    int32_t npix = r.one<int32_t>(), nt = r.one<int32_t>();
-    double dt = r.one<double>();
-    if (!r.ok || npix < 1 || nt < 1) return 103;
-    std::vector<Vector> pts, nrm;
-    std::vector<double> t0(npix), area(npix);
-    for (int i = 0; i < npix; i++) pts.push_back(r.vec());
-    r.get(t0.data(), npix);
-    r.get(area.data(), npix);
-    for (int i = 0; i < npix; i++) nrm.push_back(r.vec());
-    Vector origin = r.vec(), dx = r.vec(), dy = r.vec();
-    int32_t nx = r.one<int32_t>(), ny = r.one<int32_t>();
-    double st0 = r.one<double>(), sdt = r.one<double>(), dtx = r.one<double>(), dty = r.one<double>();
-    int32_t nots = r.one<int32_t>();
+    in.np = r.one<int32_t>();
+    in.npts = r.one<int32_t>();
+    if (!r.ok || in.np < 1 || in.npts < 1) {
+        std::fclose(f);
+        return 102;
+    }
+    size_t n = (size_t)in.np * in.npts;
+    in.a.resize(10 * n);
+    in.q.resize(in.np);
+    r.get(in.a.data(), in.a.size());
+    r.get(in.q.data(), in.q.size());
+    in.npix = r.one<int32_t>();
+    in.nt = r.one<int32_t>();
+    in.dt = r.one<double>();
+    if (!r.ok || in.npix < 1 || in.nt < 1) {
+        std::fclose(f);
+        return 103;
+    }
+    in.t0.resize(in.npix);
+    in.area.resize(in.npix);
+    for (int i = 0; i < in.npix; i++) in.pts.push_back(r.vec());
+    r.get(in.t0.data(), in.npix);
+    r.get(in.area.data(), in.npix);
+    for (int i = 0; i < in.npix; i++) in.nrm.push_back(r.vec());
+    in.origin = r.vec();
+    in.dx = r.vec();
+    in.dy = r.vec();
+    in.nx = r.one<int32_t>();
+    in.ny = r.one<int32_t>();
+    in.st0 = r.one<double>();
+    in.sdt = r.one<double>();
+    in.dtx = r.one<double>();
+    in.dty = r.one<double>();
+    in.nots = r.one<int32_t>();
     std::fclose(f);
-    if (!r.ok) return 104;
+    return r.ok ? 0 : 104;
+}
 
-    // particles: charge in e, trajectory rows [quantity][particle][point]
-    const double* t = a.data();
-    auto make = [&](int i) {
-        ChargedParticle* p = new ChargedParticle(q[i], 1.0);
-        for (int k = 0; k < npts; k++) {
-            size_t g = (size_t)i * npts + k;
-            Vector X(t[n + g], t[2 * n + g], t[3 * n + g]);
-            Vector P(t[4 * n + g], t[5 * n + g], t[6 * n + g]);
-            Vector A(t[7 * n + g], t[8 * n + g], t[9 * n + g]);
-            if (k == 0)
-                p->initTrajectory(t[g], X, P, A);
-            else
-                p->storeTrajectoryPoint(t[g], X, P, A);
+// particle i of the input: charge in e, trajectory rows [quantity][particle][point]
+ChargedParticle* make_particle(const Input& in, int i)
+{
+    const size_t n = (size_t)in.np * in.npts;
+    const double* t = in.a.data();
+    ChargedParticle* p = new ChargedParticle(in.q[i], 1.0);
+    for (int k = 0; k < in.npts; k++) {
+        size_t g = (size_t)i * in.npts + k;
+        Vector X(t[n + g], t[2 * n + g], t[3 * n + g]);
+        Vector P(t[4 * n + g], t[5 * n + g], t[6 * n + g]);
+        Vector A(t[7 * n + g], t[8 * n + g], t[9 * n + g]);
+        if (k == 0)
+            p->initTrajectory(t[g], X, P, A);
+        else
+            p->storeTrajectoryPoint(t[g], X, P, A);
+    }
+    return p;
+}
+
+// The reference's per-pixel loop (src/mesh.cpp:296-301: many threads call
+// integrateFieldTrace(point, trace) of one source, each on its own pixels)
+// against one MeshedScreen<Bunch>::integrate() of the same pixels: the
+// adapter combines the concurrent calls into batched launches, and both must
+// agree (FP64: 1e-12 of each pixel's largest field; mixed: 1e-5).  Prints the
+// time of each and how many launches the threads' calls took.
+int run_perpixel(const char* input, int n_threads, bool mixed)
+{
+    Input in;
+    if (int rc = read_input(input, in)) return rc;
+    std::unique_ptr<Bunch> all(new Bunch());
+    for (int i = 0; i < in.np; i++) all->Add(make_particle(in, i));
+    MeshedScreen<Bunch> mesh(in.pts, in.t0, in.dt, in.nt, in.area, in.nrm);
+    mesh.setSource(all.get());
+    auto t_a = std::chrono::steady_clock::now();
+    mesh.integrate();
+    const double s_batched = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_a).count();
+    std::unique_ptr<double[]> batched(mesh.getBuffer());
+    mesh.zero();
+    t_a = std::chrono::steady_clock::now();
+    std::vector<std::thread> pool;
+    std::vector<std::string> errs(n_threads);
+    for (int t = 0; t < n_threads; t++)
+        pool.emplace_back([&, t]() {
+            try {
+                for (int ip = t; ip < mesh.get_Np(); ip += n_threads)
+                    all->integrateFieldTrace(mesh.get_point(ip), mesh.get_trace(ip));
+            } catch (const std::exception& e) {
+                errs[t] = e.what();
+            }
+        });
+    for (std::thread& th : pool) th.join();
+    const double s_pixels = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_a).count();
+    for (const std::string& e : errs)
+        if (!e.empty()) throw IOexception(e);
+    std::unique_ptr<double[]> per(mesh.getBuffer());
+    int errors = 0;
+    double worst = 0.0;
+    const size_t per_pix = (size_t)6 * in.nt;
+    for (int ip = 0; ip < in.npix; ip++) {
+        double m = 0.0, d = 0.0;
+        for (size_t k = 0; k < per_pix; k++) {
+            m = std::max(m, std::fabs(batched[ip * per_pix + k]));
+            d = std::max(d, std::fabs(per[ip * per_pix + k] - batched[ip * per_pix + k]));
         }
-        return p;
-    };
+        if (m > 0) worst = std::max(worst, d / m);
+    }
+    if (!(worst <= (mixed ? 1e-5 : 1e-12))) errors++;
+    std::printf("perpixel: %d pixels, %d threads: batched %.4f s, per-pixel calls %.4f s, launches %llu, "
+                "max rel diff %.3g\n",
+                in.npix, n_threads, s_batched, s_pixels, (unsigned long long)teufel_gpu::launch_count(), worst);
+    return errors;
+}
+
+int run_screens(const char* input, const char* mesh_out, const char* screen_out)
+{
+    Input in;
+    if (int rc = read_input(input, in)) return rc;
+    const int32_t np = in.np, npix = in.npix, nt = in.nt, nx = in.nx, ny = in.ny, nots = in.nots;
+    const double dt = in.dt, st0 = in.st0, sdt = in.sdt, dtx = in.dtx, dty = in.dty;
+    const std::vector<Vector>& pts = in.pts;
+    const std::vector<Vector>& nrm = in.nrm;
+    const std::vector<double>& t0 = in.t0;
+    const std::vector<double>& area = in.area;
+    const Vector origin = in.origin, dx = in.dx, dy = in.dy;
+    auto make = [&](int i) { return make_particle(in, i); };
     // a Beam of two bunches (first half, second half) and one Bunch of all
     Beam* beam = new Beam();
     Bunch* b1 = new Bunch();
@@ -486,6 +588,9 @@ int main(int argc, char** argv)
         } else if (mode == "diffraction" && argc >= 5) {
             if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
             errors = run_diffraction(argv[2], argv[3], argv[4]);
+        } else if (mode == "perpixel" && argc >= 4) {
+            if (set_precision(argc > 4 ? argv[4] : nullptr)) return 100;
+            errors = run_perpixel(argv[2], std::atoi(argv[3]), argc > 4 && std::strcmp(argv[4], "mixed") == 0);
         } else if (mode == "screens" && argc >= 5) {
             if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
             errors = run_screens(argv[2], argv[3], argv[4]);

@@ -97,17 +97,19 @@ def _small_bunch(n_particles=48, n_steps=4000, npx=16, n_bins=256, seed=77):
     return tr, obs, t0, dt, nrm
 
 
-@pytest.mark.parametrize("window,chunk,groups,walk", [(8, 0, 1, 2), (4, 7, 3, 2), (16, 1000, 2, 1), (32, 16, 5, 2),
-                                                      (6, 0, 0, 1), (12, 0, 0, 2), (5, 3, 4, 1)])
-def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk, groups, walk):
-    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d groups=%d walk=%d" % (window, chunk, groups, walk))
+@pytest.mark.parametrize("window,chunk,groups,walk,spill", [
+    (8, 0, 1, 2, 0), (4, 7, 3, 2, 0), (16, 1000, 2, 1, 0), (32, 16, 5, 2, 2), (6, 0, 0, 1, 0), (12, 0, 0, 2, 2),
+    (5, 3, 4, 1, 0), (2, 5, 7, 2, 1), (1, 0, 9, 2, 1), (3, 48, 1, 2, 2)])
+def test_bunch_fp64_vs_oracle_launch_shapes(ctx, window, chunk, groups, walk, spill):
+    say("48 e- x 4000 steps, 16x16 px, FP64, window=%d chunk=%d groups=%d walk=%d spill=%d"
+        % (window, chunk, groups, walk, spill))
     tr, obs, t0, dt, _ = _small_bunch()
     ref = oracle.integrate_screen(tr, obs, t0, dt, 256)
     ctx.remove_screens()
     ctx.set_trajectories(tr)
     sid = ctx.add_screen(obs, t0, dt, 256)
     ctx.integrate(TG_FP64, tuning={"bins_per_window": window, "particles_per_chunk": chunk,
-                                   "window_groups": groups, "walk": walk})
+                                   "window_groups": groups, "walk": walk, "spill": spill})
     err = trace_rel_error(ctx.read_traces(sid), ref)
     print("eps_E = %.3e" % err)
     assert err <= FP64_TOL
@@ -172,8 +174,10 @@ def test_screen_spectra_match_oracle(ctx, n_bins):
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
-def test_mixed_energy_error_long_sums(ctx):
-    say("600 e- x 4000 steps in one chunk, 8x8 px, 64 coarse bins: long FP32 sums per cell; mixed vs FP64")
+@pytest.mark.parametrize("spill", [1, 2])
+def test_mixed_energy_error_long_sums(ctx, spill):
+    say("600 e- x 4000 steps in one chunk, 8x8 px, 64 coarse bins: long FP32 sums per cell; mixed vs FP64, "
+        "spill=%d" % spill)
     tr, obs, t0, dt, nrm = _small_bunch(n_particles=600, npx=8, n_bins=64)
     ctx.remove_screens()
     ctx.set_trajectories(tr)
@@ -181,19 +185,20 @@ def test_mixed_energy_error_long_sums(ctx):
     ctx.integrate(TG_FP64)
     S64 = ctx.screen_poynting(sid)
     ctx.zero_screen(sid)
-    ctx.integrate(TG_MIXED, tuning={"particles_per_chunk": 600, "bins_per_window": 8})
+    ctx.integrate(TG_MIXED, tuning={"particles_per_chunk": 600, "bins_per_window": 8, "spill": spill})
     Sm = ctx.screen_poynting(sid)
     epsW, excl = energy_rel_error(W.energy_density(Sm, nrm), W.energy_density(S64, nrm))
     print("mixed: eps_W = %.3e (excluded %.3f)" % (epsW, excl))
     assert epsW <= MIXED_ENERGY_TOL
 
 
-@pytest.mark.parametrize("prec,walk", [(TG_FP64, 2), (TG_MIXED, 2), (TG_FP64, 1), (TG_MIXED, 1)])
-def test_persistent_waves_and_lane_offsets_bitwise(ctx, prec, walk, monkeypatch):
-    say("few persistent waves (many items each), per-lane offsets off: bitwise equal; prec=%d walk=%d"
-        % (prec, walk))
+@pytest.mark.parametrize("prec,walk,spill", [(TG_FP64, 2, 1), (TG_MIXED, 2, 1), (TG_FP64, 2, 2), (TG_MIXED, 2, 2),
+                                             (TG_FP64, 1, 0), (TG_MIXED, 1, 0)])
+def test_persistent_waves_and_lane_offsets_bitwise(ctx, prec, walk, spill, monkeypatch):
+    say("few persistent waves (many items each), per-lane offsets off: bitwise equal; prec=%d walk=%d spill=%d"
+        % (prec, walk, spill))
     tr, obs, t0, dt, _ = _small_bunch()
-    tun = {"bins_per_window": 4, "particles_per_chunk": 7, "window_groups": 3, "walk": walk}
+    tun = {"bins_per_window": 4, "particles_per_chunk": 7, "window_groups": 3, "walk": walk, "spill": spill}
     ctx.remove_screens()
     ctx.set_trajectories(tr)
     sid = ctx.add_screen(obs, t0, dt, 256)

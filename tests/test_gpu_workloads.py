@@ -114,7 +114,7 @@ def test_long_sums_vs_oracle(ctx, name, n_particles, per_screen):
     ref = oracle_subset(tr, w, per_screen)
     ctx.integrate(TG_FP64)
     check(ctx, w, ids, ref, TG_FP64, name + " fp64")
-    for tun in (None, {"particles_per_chunk": n_particles}):
+    for tun in (None, {"particles_per_chunk": n_particles, "spill": 1}, {"particles_per_chunk": n_particles, "spill": 2}):
         for i in ids:
             ctx.zero_screen(i)
         ctx.integrate(TG_MIXED, tuning=tun)

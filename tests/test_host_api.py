@@ -330,3 +330,34 @@ def test_host_screens_vs_oracle(tmp_path, precision):
     else:
         n = np.array([0.0, 0.0, -1.0])
         assert energy_rel_error(pixel_energy(straces, sdt, n), pixel_energy(sref, sdt, n))[0] <= 1e-5
+
+
+def _perpixel_input(path, n_particles=8, n_steps=2000, npx=16, n_bins=128):
+    w = W.make("c2", n_particles=n_particles, n_steps=n_steps, seed=41)
+    u = w.und
+    tr = oracle.track_undulator(w.bunch.x0, w.bunch.u0, w.dt, w.n_steps, u.bpeak, u.lambda_u, u.n_periods,
+                                u.origin)
+    obs, nrm, area = W.grid_screen(npx, 0.2, 10.0)
+    t_c, x_c = W._centre(u, 60.0)
+    t0, dt = W.set_windows(obs, n_bins, u, 60.0, t_c, x_c, 0.3e-3)
+    scr = dict(origin=[0.0, 0.0, 10.0], dx=[0.01, 0.0, 0.0], dy=[0.0, 0.01, 0.0], nx=2, ny=2, t0=float(t0[0]),
+               dt=dt, dtx=0.0, dty=0.0, nots=n_bins)
+    write_input(path, tr, obs, t0, dt, n_bins, area, nrm, scr)
+    return obs.shape[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_per_pixel_calls_are_combined(tmp_path, precision):
+    """The reference's per-pixel loop, 16 threads calling
+    Bunch::integrateFieldTrace(point, trace) at once for 256 pixels, gives the
+    batched MeshedScreen::integrate() result (FP64 1e-12, mixed 1e-5 of each
+    pixel's largest field), and the adapter combines the calls into far fewer
+    launches than pixels."""
+    inp = str(tmp_path / "in.bin")
+    n_pix = _perpixel_input(inp)
+    r = run("perpixel", inp, "16", precision)
+    assert r.returncode == 0
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("perpixel:")][0]
+    launches = int(re.search(r"launches (\d+)", line).group(1))
+    assert launches <= 1 + n_pix // 2, line

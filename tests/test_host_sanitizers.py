@@ -67,6 +67,39 @@ def test_no_device_is_an_ioexception(program):
     assert rc == IOEXCEPTION and not rep and "tg_create" in out, out[-3000:]
 
 
+def test_concurrent_calls_without_device(program, tmp_path):
+    """16 threads calling integrateFieldTrace at once: with no device every
+    combined launch fails, and every caller gets the IOexception (exit status
+    110), with no sanitizer report and no hang."""
+    from test_host_api import _perpixel_input
+    inp = str(tmp_path / "in.bin")
+    _perpixel_input(inp, n_particles=2, n_steps=50, npx=8, n_bins=16)
+    rc, rep, out = run(program, "perpixel", inp, "16", "fp64")
+    assert rc == IOEXCEPTION and not rep and "tg_create" in out, out[-3000:]
+
+
+def test_thread_sanitizer_on_combined_calls(tmp_path):
+    """The same concurrent calls under ThreadSanitizer (host code and the
+    device-less stand-in of the C ABI): no data race in the adapter's request
+    combining."""
+    cxx = shutil.which(os.environ.get("CXX", "g++"))
+    if not cxx:
+        pytest.skip("no C++ compiler")
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "host_api_test_tsan")
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    r = subprocess.run([cxx, "-std=c++14", "-O1", "-g", "-fsanitize=thread", "-I" + HOST, "-o", exe] + srcs + [
+        os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp"), os.path.join(ROOT, "tests", "cpp", "tg_stub.cpp")],
+        capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    from test_host_api import _perpixel_input
+    inp = str(tmp_path / "in.bin")
+    _perpixel_input(inp, n_particles=2, n_steps=50, npx=8, n_bins=16)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1:exitcode=68")
+    r = subprocess.run([exe, "perpixel", inp, "16", "fp64"], capture_output=True, text=True, timeout=120, env=env,
+                       cwd="/tmp")
+    assert r.returncode == IOEXCEPTION and "ThreadSanitizer" not in r.stderr, (r.stdout + r.stderr)[-3000:]
+
+
 def _corrupt_runs(program, tmp_path, src, mode, n_trunc, n_flip, seed):
     data = open(src, "rb").read()
     rng = np.random.default_rng(seed)

@@ -195,7 +195,7 @@ class RadiationContext:
         t = TgTuning()
         self._chk(self.lib.tg_autotune(self.h, precision, ctypes.byref(t)))
         return {"bins_per_window": t.bins_per_window, "particles_per_chunk": t.particles_per_chunk,
-                "window_groups": t.window_groups, "use_graph": t.use_graph}
+                "window_groups": t.window_groups, "use_graph": t.use_graph, "spill": t.spill}
 
     def read_traces(self, sid, out=None, accumulate=False):
         n_pix, n_bins = self.screens[sid]

@@ -18,7 +18,8 @@ _fp = ctypes.POINTER(ctypes.c_float)
 
 class TgTuning(ctypes.Structure):
     _fields_ = [("bins_per_window", ctypes.c_int), ("particles_per_chunk", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("window_groups", ctypes.c_int), ("walk", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("window_groups", ctypes.c_int), ("walk", ctypes.c_int),
+                ("spill", ctypes.c_int)]
 
 
 # (name, restype, argtypes) for every entry point of include/teufel_gpu.h

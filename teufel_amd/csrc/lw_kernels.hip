@@ -427,45 +427,53 @@ __device__ __forceinline__ int wave_min(int v)
 }
 
 // Deposit of step (p1 -> p2) into the window [b0, b1), FP64: retarded times
-// and bucket edges as the reference forms them.
+// and bucket edges as the reference forms them.  With spill, the window has
+// one more LDS bucket, b1: a step that ends in bucket b1 (the walk's last
+// step for the lane) is deposited there too, whole, and is not walked again
+// in the next window, which takes that bucket over as its first; a step that
+// ends beyond b1 deposits only its buckets below b1 here (the next window
+// walks it again for the rest).  Walked steps start before b1, so a common
+// (one- or two-bucket) step reaches at most b1.
 __device__ __forceinline__ void deposit_window(const Window<double>& win, bool active, const Point& p1,
                                                const Point& p2, double t0, double dt, double inv_dt, double tmin,
-                                               double tmax, int n_bins, int b0, int b1)
+                                               double tmax, int n_bins, int b0, int b1, bool spill)
 {
     const bool valid = active && p2.ts > tmin && p1.ts < tmax;    // particle.cpp:793
     const bool case_a = p1.ts <= tmin;
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
     const TwoBuckets<double> t = step_common<double>(RcpRn{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
-                                                     inv_dt, n_bins, b0, b1, -1, -1);
+                                                     inv_dt, n_bins, b0, b1 + (spill ? 1 : 0), -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
             WindowAcc<double> acc{win, b0, p1.f, p2.f};
             step_general(acc, rcp_rn(p2.ts - p1.ts), case_a, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt, n_bins,
-                         b0, b1);
+                         b0, b1 + ((spill && i2 == b1) ? 1 : 0));
         }
     }
 }
 
 // Mixed: the same rule in bucket units (deposit_step_units), the common
-// step branch-free with FP32 weights, the general step under a ballot.
+// step branch-free with FP32 weights, the general step under a ballot; spill
+// as for FP64.
 __device__ __forceinline__ void deposit_window(const Window<float>& win, bool active, const PointM& p1,
                                                const PointM& p2, double, double, double, double, double,
-                                               int n_bins, int b0, int b1)
+                                               int n_bins, int b0, int b1, bool spill)
 {
     const bool valid = active && p2.idx >= 0 && p1.idx < n_bins;  // particle.cpp:793
     const bool case_a = p1.idx < 0;                               // particle.cpp:796
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
-    const TwoBuckets<float> t =
-        step_common_units<float>(RcpF32{}, valid && !gen, p1.idx, p1.u, p2.idx, p2.u, i2, b0, b1, -1, -1);
+    const TwoBuckets<float> t = step_common_units<float>(RcpF32{}, valid && !gen, p1.idx, p1.u, p2.idx, p2.u, i2,
+                                                         b0, b1 + (spill ? 1 : 0), -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
             WindowAcc<float> acc{win, b0, p1.f, p2.f};
             const double q1 = (double)p1.idx + (double)p1.u, q2 = (double)p2.idx + (double)p2.u;
-            step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0, b1);
+            step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0,
+                         b1 + ((spill && i2 == b1) ? 1 : 0));
         }
     }
 }
@@ -632,7 +640,7 @@ __device__ __forceinline__ int walk_particle(const char* __restrict__ rp, int k,
             break;
         }
         eval_point<PREC>(src.next(k + 1), Px, Py, Pz, t0, dt, inv_dt, p1);
-        deposit_window(win, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        deposit_window(win, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, false);
         ++steps;
         if (next_k < 0 && __ballot(active && p1.idx >= b1)) {
             next_k = k;
@@ -690,18 +698,25 @@ __device__ __forceinline__ int point_index(const RecBuffer<typename RecOf<PREC>:
 // from k to the step it stopped at in the previous window, saturated at 255
 // (every point between lies before b0, so the skip would have passed it).
 // lc_out (when not null) receives the offsets of this walk's stopping steps
-// from nk.
+// from nk.  A searched start, and a stored offset below the saturation, is
+// exactly where the skip would stop (the point after it lies at or beyond
+// b0), so only lanes starting at the wave's cursor or at a saturated offset
+// skip.  With spill (deposit_window), a lane whose last step ends in bucket
+// b1 has deposited it whole and stops at the step after it; carry says that
+// the previous window of the item spilled, so that the skip passes such a
+// step.
 template <int PREC>
 __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, int NP,
                                           bool active, double Px, double Py, double Pz, double t0, double dt,
                                           double inv_dt, double tmin, double tmax, int n_bins, int b0, int b1,
                                           const Window<typename RecOf<PREC>::amp>& win, bool search,
                                           const unsigned char* __restrict__ lc_in, unsigned char* __restrict__ lc_out,
-                                          int& nk, int& nmin)
+                                          bool spill, bool carry, int& nk, int& nmin)
 {
     using Rec = typename RecOf<PREC>::type;
     const RecBuffer<Rec> rbuf(rp, NP);
     int kl = k;
+    bool exact = search;
     if (search) {
         int lo = 1, hi = NP;  // first point in [lo, hi) with bucket >= b0; NP: none
         bool open = active && lo < hi;
@@ -717,13 +732,20 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
         }
         kl = min(lo - 1, NP - 1);
     } else if (lc_in) {
-        kl = min(k + (int)lc_in[threadIdx.x], NP - 1);
+        const int off = lc_in[threadIdx.x];
+        kl = min(k + off, NP - 1);
+        exact = off < 255;
     }
-    bool skip = active && kl < NP - 1;
+    // With carry (the previous window spilled), a step that starts before b0
+    // and ends in bucket b0 was deposited whole there: skipped too.
+    bool skip = active && kl < NP - 1 && !exact;
+    int prev = (skip && carry) ? point_index<PREC>(rbuf, kl, Px, Py, Pz, t0, dt, inv_dt) : INT_MIN;
     while (__ballot(skip)) {
         if (skip) {
-            if (point_index<PREC>(rbuf, kl + 1, Px, Py, Pz, t0, dt, inv_dt) < b0) {
+            const int nxt = point_index<PREC>(rbuf, kl + 1, Px, Py, Pz, t0, dt, inv_dt);
+            if (nxt < b0 || (carry && nxt == b0 && prev < b0)) {
                 ++kl;
+                prev = nxt;
                 skip = kl < NP - 1;
             } else {
                 skip = false;
@@ -757,11 +779,15 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
         __builtin_amdgcn_sched_barrier(0);
         rnext = rbuf.load(kl + 2);
         eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
-        deposit_window(win, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
+        deposit_window(win, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, spill);
         ++steps;
         if (walking) {
             if (pe.idx >= b1) {
                 walking = false;
+                if (spill && pe.idx == b1) {   // deposited whole, into the spill bucket
+                    ++kl;
+                    idx0 = pe.idx;
+                }
             } else {
                 ++kl;
                 idx0 = pe.idx;
@@ -866,28 +892,57 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
             }
         }
     }
+    // Spill bucket (per-lane walk): a window that is not the item's last has
+    // one more LDS bucket, its b1, which receives the walks' last steps that
+    // end there (deposit_window); the next window starts with it as its bucket
+    // 0.  carry: bucket 0 of this window holds the previous window's spill
+    // bucket; carry_stored: part of that bucket is already in the partial
+    // trace (the mixed kernel flushes the spill bucket with the window);
+    // carry_terms: the terms it may hold since.
+    bool carry = false, carry_stored = false;
+    unsigned carry_terms = 0;
     for (int b0 = it.bin_lo; b0 < it.bin_hi; b0 += W) {
         const int b1 = b0 + W < it.bin_hi ? b0 + W : it.bin_hi;
-        const int ncell = (b1 - b0) * 6;
-        double* out = S.partial + (((size_t)it.chunk * S.n_pix + (active ? pix : 0)) * nb + b0) * 6;
-        for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
+        const int nw = b1 - b0;
+        const bool spill = LANES && S.spill && b1 < it.bin_hi;
+        const int ncell = (nw + (spill ? 1 : 0)) * 6;
+        // chunk 0 adds into the trace store, the others store partial traces
+        const size_t px = (size_t)(active ? pix : 0);
+        double* out = it.chunk == 0 ? S.traces + (px * nb + b0) * 6
+                                    : S.partial + (((size_t)(it.chunk - 1) * S.n_pix + px) * nb + b0) * 6;
+        if (carry) {
+            // the previous window was a full one (only those spill): its
+            // spill bucket is bucket W
+#pragma unroll
+            for (int c = 0; c < 6; ++c) win.lane_base[c * kLanes] = win.lane_base[(6 * W + c) * kLanes];
+            for (int j = 6; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
+        } else {
+            for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
+        }
         bool first_flush = true;
-        bool dirty = false;    // the window holds terms not yet in the partial trace
-        unsigned terms = 0;    // estimated terms per cell since the last flush
+        bool b0_stored = carry && carry_stored;   // out of bucket 0 already holds part of it
+        bool spill_stored = false;                // out of the spill bucket (bucket nw) holds part of it
+        bool dirty = carry;    // the window holds terms not yet in the partial trace
+        unsigned terms = carry ? carry_terms : 0;  // estimated terms per cell since the last flush
         // cell j of the window as an FP64 trace value (mixed cells hold c B
         // for the magnetic components)
         auto cell = [&](int j, int c) {
             const double v = (double)win.lane_base[(6 * j + c) * kLanes];
             return (PREC == 1 && c >= 3) ? v * kInvSpeedOfLight : v;
         };
+        auto store = [&](int j, bool add) {
+            add = add || it.chunk == 0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[6 * j + c] = add ? out[6 * j + c] + cell(j, c) : cell(j, c);
+        };
         auto flush = [&]() {
             if (active) {
-                for (int j = 0; j < b1 - b0; ++j)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) out[6 * j + c] = first_flush ? cell(j, c) : out[6 * j + c] + cell(j, c);
+                for (int j = 0; j < nw; ++j) store(j, !first_flush || (j == 0 && b0_stored));
+                if (spill) store(nw, spill_stored);
             }
             for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
             first_flush = false;
+            spill_stored = spill_stored || spill;
             dirty = false;
             terms = 0;
         };
@@ -908,9 +963,9 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                     // per-lane offsets: read when a previous window of this
                     // item stored them, written when a next window follows
                     unsigned char* lc = lane_cur ? lane_cur + (size_t)(p - it.p0) * kLanes : nullptr;
-                    ws = walk_lanes<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin, tmax,
-                                          nb, b0, b1, win, first, first ? nullptr : lc,
-                                          b1 < it.bin_hi ? lc : nullptr, nk, nmin);
+                    ws = walk_lanes<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt,
+                                          inv_dt, tmin, tmax, nb, b0, b1, win, first, first ? nullptr : lc,
+                                          b1 < it.bin_hi ? lc : nullptr, spill, carry, nk, nmin);
                 } else {
                     const bool verify = first && k > 0;
                     ws = walk_particle<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin,
@@ -921,23 +976,24 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                 if (lane == j) ncur = make_int2(nk, nmin);
                 if constexpr (PREC == 1) {
                     dirty = true;
-                    terms += (unsigned)ws / (unsigned)(b1 - b0) + 1;
+                    terms += (unsigned)ws / (unsigned)nw + 1;
                     if (terms >= kFlushTerms) flush();
                 }
             }
             if (b1 < it.bin_hi && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
         }
+        // the window's buckets into the partial trace; the spill bucket stays
+        // in LDS for the next window
         if (active) {
             if (PREC == 0 || first_flush) {
-                for (int j = 0; j < b1 - b0; ++j)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) out[6 * j + c] = cell(j, c);
+                for (int j = 0; j < nw; ++j) store(j, j == 0 && b0_stored);
             } else if (dirty) {
-                for (int j = 0; j < b1 - b0; ++j)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) out[6 * j + c] += cell(j, c);
+                for (int j = 0; j < nw; ++j) store(j, true);
             }
         }
+        carry = spill;
+        carry_stored = spill_stored;
+        carry_terms = terms;
     }
 }
 
@@ -984,7 +1040,8 @@ template __global__ void radiation_kernel<1, true>(const WorkItem*, int, int*, c
                                                     unsigned char*, size_t, int, int, unsigned long long*, int);
 
 // ---------------------------------------------------------------------------
-// Sum the chunk partials into the trace store, chunks in order.
+// Add the partial traces of chunks 1.. into the trace store (which chunk 0
+// added into), chunks in order.
 // ---------------------------------------------------------------------------
 __global__ void reduce_partials_kernel(const double* __restrict__ partial, double* __restrict__ out,
                                        size_t n, int n_chunks)

@@ -57,6 +57,7 @@ struct LaunchPlan {
     int max_window = 0;
     int groups = 1;
     bool lanes = true;  // per-lane walk (tg_tuning.walk == 2)
+    bool spill = true;  // windows with a spill bucket (per-lane walk)
     bool lane_cursors = false;  // per-lane cursor offsets kept between windows
     size_t lds = 0;             // dynamic LDS per wave
 };
@@ -188,7 +189,7 @@ int alloc_traj(tg_context* c, int n_particles, int n_points)
     c->n_points = n_points;
     c->traj_set = true;
     // the tuned chunk and group counts were timed on the previous set
-    for (tg_tuning& t : c->tuned) t.particles_per_chunk = t.window_groups = 0;
+    for (tg_tuning& t : c->tuned) t.particles_per_chunk = t.window_groups = t.spill = 0;
     return TG_OK;
 }
 
@@ -379,17 +380,24 @@ int default_window(const tg_context* c, int n_bins)
 size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
 
 // LDS of one wave: window x 6 components x 64 lanes, plus the FP64
-// wave-uniform walk's record chunks.
+// wave-uniform walk's record chunks; the per-lane walk's windows have one
+// more bucket, the spill bucket (run_item).
 size_t stage_bytes(int prec, bool lanes)
 {
     if (lanes) return 0;  // the per-lane walk reads records straight from global memory
     return (prec == TG_FP64 ? SourceOf<0>::lds_doubles : SourceOf<1>::lds_doubles) * sizeof(double);
 }
 
-int max_window_for_lds(const tg_context* c, int prec, bool lanes)
+size_t lds_per_wave(int prec, bool lanes, bool spill, int window)
 {
-    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec)));
+    return (size_t)(window + (spill ? 1 : 0)) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, lanes);
 }
+
+int max_window_for_lds(const tg_context* c, int prec, bool lanes, bool spill)
+{
+    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec))) - (spill ? 1 : 0);
+}
+
 
 // tg_tuning.walk: 1 = wave-uniform walk, 2 = per-lane walk; 0 picks the
 // context default (2, or the TG_WALK environment variable at tg_create).
@@ -397,6 +405,12 @@ bool lane_walk(const tg_context* c, const tg_tuning* tun)
 {
     int w = (tun && tun->walk > 0) ? tun->walk : c->default_walk;
     return w != 1;
+}
+
+// The spill bucket applies to the per-lane walk; tg_tuning.spill 2 turns it off.
+bool spill_on(const tg_context* c, const tg_tuning* tun)
+{
+    return lane_walk(c, tun) && !(tun && tun->spill == 2);
 }
 
 // Estimated trajectory steps of one particle whose arrival falls into
@@ -443,12 +457,13 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
         n_chunks = (int)std::max<long long>(1, (target + total_tiles - 1) / total_tiles);
         n_chunks = std::min(n_chunks, std::max(1, np / 32));
     }
-    // cap the partial-trace scratch at 16 GB (work items get finer through
-    // window groups, which cost no memory)
+    // cap the partial-trace scratch (chunks 1..; chunk 0 adds into the trace
+    // store) at 16 GB (work items get finer through window groups, which cost
+    // no memory)
     size_t per_chunk = 0;
     for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
     const size_t cap = (size_t)16 << 30;
-    while (n_chunks > 1 && (size_t)n_chunks * per_chunk * sizeof(double) > cap) --n_chunks;
+    while (n_chunks > 1 && (size_t)(n_chunks - 1) * per_chunk * sizeof(double) > cap) --n_chunks;
     n_chunks = std::max(1, std::min(n_chunks, np));
     int chunk = (np + n_chunks - 1) / n_chunks;
     n_chunks = (np + chunk - 1) / chunk;
@@ -456,7 +471,8 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     plan.chunk = chunk;
     plan.screen_ids = ids;
     plan.lanes = lane_walk(c, tun);
-    int wmax_lds = max_window_for_lds(c, prec, plan.lanes);
+    plan.spill = spill_on(c, tun);
+    int wmax_lds = max_window_for_lds(c, prec, plan.lanes, plan.spill);
     size_t part_off = 0;
     for (int id : ids) {
         ScreenHost& s = c->screens[id];
@@ -464,6 +480,7 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
         d.obs = s.d_obs;
         d.t0 = s.d_t0;
         d.perm = s.d_perm;
+        d.traces = s.d_traces;
         d.partial = nullptr;  // patched once the scratch is allocated
         d.dt = s.dt;
         d.inv_dt = 1.0 / s.dt;
@@ -474,9 +491,10 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
                                                    : (s.window[prec] > 0 ? s.window[prec] : default_window(c, s.n_bins));
         w = std::max(1, std::min({w, s.n_bins, wmax_lds}));
         d.window = w;
+        d.spill = plan.spill ? 1 : 0;
         plan.max_window = std::max(plan.max_window, w);
         plan.sdev.push_back(d);
-        part_off += (size_t)n_chunks * s.n_pix * s.n_bins * 6;
+        part_off += (size_t)(n_chunks - 1) * s.n_pix * s.n_bins * 6;
     }
     plan.partial_doubles = part_off;
     // Window groups: each (tile, chunk) is split into groups of consecutive
@@ -485,7 +503,7 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     // launch short; a group seeds its cursors by search instead of walking
     // from step 0.
     long long base_items = total_tiles * n_chunks;
-    const size_t lds_wave = (size_t)plan.max_window * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, plan.lanes);
+    const size_t lds_wave = lds_per_wave(prec, plan.lanes, plan.spill, plan.max_window);
     long long slots = (long long)std::max(c->n_cu, 1) * (long long)std::max<size_t>(1, c->max_lds / lds_wave);
     int groups = (int)std::max<long long>(1, std::min<long long>(64, (16 * slots + base_items - 1) / base_items));
     if (tun && tun->window_groups > 0) groups = tun->window_groups;
@@ -568,7 +586,7 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
     size_t off = 0;
     for (size_t i = 0; i < plan.sdev.size(); ++i) {
         plan.sdev[i].partial = c->d_partial + off;
-        off += (size_t)plan.n_chunks * plan.sdev[i].n_pix * plan.sdev[i].n_bins * 6;
+        off += (size_t)(plan.n_chunks - 1) * plan.sdev[i].n_pix * plan.sdev[i].n_bins * 6;
     }
     TG_HIP(c, hipMemcpy(c->d_items, plan.items.data(), plan.items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     TG_HIP(c, hipMemcpy(c->d_sdev, plan.sdev.data(), plan.sdev.size() * sizeof(ScreenDev), hipMemcpyHostToDevice));
@@ -603,11 +621,11 @@ int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 #undef TG_LAUNCH
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
-    for (size_t i = 0; i < plan.sdev.size(); ++i) {
+    for (size_t i = 0; i < plan.sdev.size() && plan.n_chunks > 1; ++i) {
         const ScreenHost& s = c->screens[plan.screen_ids[i]];
         size_t n = (size_t)s.n_pix * s.n_bins * 6;
         unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
-        reduce_partials_kernel<<<blocks, 256, 0, c->stream>>>(plan.sdev[i].partial, s.d_traces, n, plan.n_chunks);
+        reduce_partials_kernel<<<blocks, 256, 0, c->stream>>>(plan.sdev[i].partial, s.d_traces, n, plan.n_chunks - 1);
         TG_HIP(c, hipGetLastError());
     }
     if (events) TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
@@ -642,12 +660,13 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         if (tun->particles_per_chunk > 0) eff.particles_per_chunk = tun->particles_per_chunk;
         if (tun->window_groups > 0) eff.window_groups = tun->window_groups;
         if (tun->walk > 0) eff.walk = tun->walk;
+        if (tun->spill > 0) eff.spill = tun->spill;
         eff.use_graph = tun->use_graph;
     }
     bool use_graph = eff.use_graph != 0;
     char keybuf[256];
-    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window, eff.particles_per_chunk,
-             eff.window_groups, eff.walk, screen_id, p0, p1, c->screens.size());
+    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window,
+             eff.particles_per_chunk, eff.window_groups, eff.walk, eff.spill, screen_id, p0, p1, c->screens.size());
     std::string key(keybuf);
     if (use_graph && c->gexec && c->graph_key == key) {
         TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -1217,8 +1236,8 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // the full launch's number of work items per resident wave (which sets
     // how long the launch's tail is); only the items' particle count is
     // smaller.
-    const int cands[] = {4, 5, 6, 7, 8, 9, 10, 12, 16};
-    int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
+    const int cands[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16};
+    int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr), spill_on(c, nullptr));
     double pairs_per_particle = 0;
     for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
     const double want = 1e11 / std::max(1.0, pairs_per_particle);
@@ -1256,59 +1275,78 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
                                                " failed (TG_AUTOTUNE_FAIL_TRIAL)")
                    : TG_OK;
     };
+    // One timed launch of the sample with launch shape t (its chunk given as
+    // the full launch's chunk count n_ch); the kernel time in ms, < 0 on error.
+    auto trial = [&](tg_tuning t, long long n_ch, const char* what) -> float {
+        t.particles_per_chunk = sample_chunk(n_ch);
+        if ((rc = integrate_impl(c, precision, &t, -1, 0, sample)) || (rc = injected())) return -1.f;
+        if (log)
+            std::fprintf(stderr, "[tg_autotune] %s W=%d groups=%d chunk(full)=%lld spill=%d: %.2f ms (%d particles)\n",
+                         what, t.bins_per_window, t.window_groups, (long long)((c->n_particles + n_ch - 1) / n_ch),
+                         t.spill, c->kernel_ms, sample);
+        return c->kernel_ms;
+    };
     auto trials = [&]() -> int {
+        // First stage: the window, with the default spill bucket.
         for (int w : cands) {
             if (w > wmax || w > max_bins) continue;
             tg_tuning t = {};
             t.bins_per_window = w;
             LaunchPlan full;
             if ((rc = make_plan(c, precision, ids, 0, c->n_particles, &t, full))) return rc;
-            t.particles_per_chunk = sample_chunk(full.n_chunks);
             t.window_groups = full.groups;
-            if ((rc = integrate_impl(c, precision, &t, -1, 0, sample)) || (rc = injected())) return rc;
-            if (log)
-                std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms (%d particles)\n", w, full.groups,
-                             full.chunk, c->kernel_ms, sample);
-            if (c->kernel_ms < best_ms) {
-                best_ms = c->kernel_ms;
+            const float ms = trial(t, full.n_chunks, "window");
+            if (ms < 0.f) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
                 best_w = w;
                 best_full = full;
             }
         }
         // Second stage, at the chosen window: the number of window groups (item
-        // granularity against the search each group costs) and the particle
+        // granularity against the search each group costs), the particle
         // chunk (items per tile against partial traces to reduce), each halved
-        // and doubled around the default.  A shape replaces the default only
-        // when it is more than 2 % faster, so timing noise does not move it.
+        // and doubled around the default, and the window without its spill
+        // bucket (one bucket less of LDS per wave, so more waves, against
+        // walking every window's crossing steps twice), at the same window and
+        // one wider.  A shape replaces the default only when it is more than
+        // 2 % faster, so timing noise does not move it.
         if (best_full.chunk > 0) {
             const int g0 = best_full.groups, ch0 = best_full.chunk;
-            int best_g = g0, best_ch = ch0;
+            tg_tuning pick = {};
+            pick.bins_per_window = best_w;
             const float base_ms = best_ms;
             float shape_ms = base_ms;
             size_t per_chunk = 0;
             for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
-            const int shapes[4][2] = {{g0 / 2, ch0}, {g0 * 2, ch0}, {g0, ch0 * 2}, {g0, ch0 / 2}};
+            const int spill_w1 = std::min(best_w + 1, std::min(max_bins, max_window_for_lds(c, precision, true, false)));
+            const int shapes[6][4] = {{best_w, g0 / 2, ch0, 0}, {best_w, g0 * 2, ch0, 0}, {best_w, g0, ch0 * 2, 0},
+                                      {best_w, g0, ch0 / 2, 0}, {best_w, g0, ch0, 2},     {spill_w1, g0, ch0, 2}};
             for (const auto& sh : shapes) {
-                const int g = sh[0], ch = sh[1];
+                const int w = sh[0], g = sh[1], ch = sh[2], sp = sh[3];
                 if (g < 1 || g > 64 || ch < 1 || ch > c->n_particles) continue;
+                if (sp == 2 && !lane_walk(c, nullptr)) continue;
                 const size_t n_ch = ((size_t)c->n_particles + ch - 1) / ch;
-                if (n_ch * per_chunk * sizeof(double) > ((size_t)16 << 30)) continue;
+                if ((n_ch - 1) * per_chunk * sizeof(double) > ((size_t)16 << 30)) continue;
                 tg_tuning t = {};
-                t.bins_per_window = best_w;
-                t.particles_per_chunk = sample_chunk((long long)n_ch);
+                t.bins_per_window = w;
                 t.window_groups = g;
-                if ((rc = integrate_impl(c, precision, &t, -1, 0, sample)) || (rc = injected())) return rc;
-                if (log)
-                    std::fprintf(stderr, "[tg_autotune] W=%d groups=%d chunk=%d: %.2f ms\n", best_w, g, ch, c->kernel_ms);
-                if (c->kernel_ms < shape_ms) {
-                    shape_ms = c->kernel_ms;
-                    best_g = g;
-                    best_ch = ch;
+                t.spill = sp;
+                const float ms = trial(t, (long long)n_ch, "shape");
+                if (ms < 0.f) return rc;
+                if (ms < shape_ms) {
+                    shape_ms = ms;
+                    pick = t;
+                    pick.particles_per_chunk = ch;
                 }
             }
-            if (shape_ms < 0.98f * base_ms && (best_g != g0 || best_ch != ch0)) {
-                best.window_groups = best_g;
-                best.particles_per_chunk = best_ch;
+            if (shape_ms < 0.98f * base_ms) {
+                best_w = pick.bins_per_window;
+                best.spill = pick.spill;
+                if (pick.window_groups != g0 || pick.particles_per_chunk != ch0) {
+                    best.window_groups = pick.window_groups;
+                    best.particles_per_chunk = pick.particles_per_chunk;
+                }
             }
         }
         return TG_OK;

@@ -88,18 +88,22 @@ __host__ __device__ constexpr unsigned rec_offset(int k)
            ((unsigned)k % kRecBlock<Rec>) * 16u;
 }
 
-// One screen as the kernel sees it.
+// One screen as the kernel sees it.  Particle chunk 0 of a launch adds its
+// fields straight into the trace store; chunks 1.. write partial traces that
+// a second kernel adds, in chunk order, after the launch.
 struct ScreenDev {
     const double* obs;    // [n_pix][3]
     const double* t0;     // [n_pix]
     const int* perm;      // [n_tiles*64] pixel index per lane, -1 = padding
-    double* partial;      // [n_chunks][n_pix][n_bins][6]
+    double* traces;       // [n_pix][n_bins][6], the trace store (chunk 0)
+    double* partial;      // [n_chunks - 1][n_pix][n_bins][6] (chunks 1..)
     double dt;
     double inv_dt;        // RN(1/dt)
     int n_pix;
     int n_bins;
     int n_tiles;
     int window;           // trace bins per LDS window
+    int spill;            // per-lane walk: windows hold a spill bucket (run_item)
 };
 
 // One wavefront's work: a pixel tile of one screen and a chunk of particles.

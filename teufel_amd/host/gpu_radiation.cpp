@@ -1,8 +1,10 @@
 // gpu_radiation.cpp - host adapter over the C ABI.
 #include "gpu_radiation.h"
 
+#include <condition_variable>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -14,8 +16,26 @@
 namespace teufel_gpu {
 namespace {
 
+// One integrate() call waiting to be launched.
+struct Request {
+    const std::vector<ChargedParticle*>* particles;
+    const std::vector<Vector>* points;
+    const std::vector<FieldTrace*>* traces;
+    bool done = false;
+    std::exception_ptr error;
+};
+
 struct State {
     std::mutex mu;
+    // integrate() requests from concurrent callers (the reference's
+    // per-pixel OpenMP loop calls integrateFieldTrace from many threads,
+    // src/mesh.cpp:296-301): one caller at a time launches everything queued
+    // so far, one batched launch per particle list, while the others wait.
+    std::mutex qmu;
+    std::condition_variable qcv;
+    std::vector<Request*> queue;
+    bool launching = false;
+    unsigned long long launches = 0;   // batched launches made (under mu)
     tg_context* ctx = nullptr;
     int device = -1;
     int precision = -1;
@@ -122,12 +142,12 @@ void set_device(int device)
     s.device = device;
 }
 
-void integrate(const std::vector<ChargedParticle*>& particles, const std::vector<Vector>& points,
-               const std::vector<FieldTrace*>& traces)
+namespace {
+
+// One batched launch for points/traces over particles (state mutex held).
+void integrate_locked(State& s, const std::vector<ChargedParticle*>& particles, const std::vector<Vector>& points,
+                      const std::vector<FieldTrace*>& traces)
 {
-    if (points.size() != traces.size()) throw IOexception("teufel_gpu::integrate - points/traces size mismatch.");
-    State& s = state();
-    std::lock_guard<std::mutex> g(s.mu);
     tg_context* ctx = context(s);
     upload(s, particles);
     check(ctx, tg_remove_screens(ctx), "tg_remove_screens");
@@ -152,6 +172,7 @@ void integrate(const std::vector<ChargedParticle*>& particles, const std::vector
         screens.emplace_back(sid, &c.second);
     }
     check(ctx, tg_integrate(ctx, s.precision, nullptr), "tg_integrate");
+    ++s.launches;
     for (const auto& sc : screens) {
         const std::vector<size_t>& idx = *sc.second;
         size_t nb = traces[idx[0]]->get_N();
@@ -164,6 +185,71 @@ void integrate(const std::vector<ChargedParticle*>& particles, const std::vector
         }
     }
     check(ctx, tg_remove_screens(ctx), "tg_remove_screens");
+}
+
+// Launch a batch of queued requests: requests over the same particle list
+// (the common case, every pixel of an observer loop over one source) share
+// one launch; each request's traces receive its fields, and a request whose
+// launch failed gets that launch's exception.
+void launch_batch(State& s, const std::vector<Request*>& batch)
+{
+    std::lock_guard<std::mutex> g(s.mu);
+    std::vector<bool> taken(batch.size(), false);
+    for (size_t i = 0; i < batch.size(); i++) {
+        if (taken[i]) continue;
+        std::vector<size_t> group;
+        for (size_t j = i; j < batch.size(); j++)
+            if (!taken[j] && (j == i || *batch[j]->particles == *batch[i]->particles)) {
+                taken[j] = true;
+                group.push_back(j);
+            }
+        std::vector<Vector> pts;
+        std::vector<FieldTrace*> trs;
+        for (size_t j : group) {
+            pts.insert(pts.end(), batch[j]->points->begin(), batch[j]->points->end());
+            trs.insert(trs.end(), batch[j]->traces->begin(), batch[j]->traces->end());
+        }
+        std::exception_ptr err;
+        try {
+            integrate_locked(s, *batch[i]->particles, pts, trs);
+        } catch (...) {
+            err = std::current_exception();
+        }
+        for (size_t j : group) batch[j]->error = err;
+    }
+}
+
+}  // namespace
+
+void integrate(const std::vector<ChargedParticle*>& particles, const std::vector<Vector>& points,
+               const std::vector<FieldTrace*>& traces)
+{
+    if (points.size() != traces.size()) throw IOexception("teufel_gpu::integrate - points/traces size mismatch.");
+    for (const FieldTrace* t : traces)
+        if (!t) throw IOexception("teufel_gpu::integrate - null FieldTrace.");
+    State& s = state();
+    Request r{&particles, &points, &traces};
+    std::unique_lock<std::mutex> lk(s.qmu);
+    s.queue.push_back(&r);
+    while (!r.done) {
+        if (s.launching) {
+            s.qcv.wait(lk);
+            continue;
+        }
+        // this caller launches everything queued until now, its own request
+        // included; requests queued meanwhile go into the next batch
+        s.launching = true;
+        std::vector<Request*> batch;
+        batch.swap(s.queue);
+        lk.unlock();
+        launch_batch(s, batch);
+        lk.lock();
+        for (Request* q : batch) q->done = true;
+        s.launching = false;
+        s.qcv.notify_all();
+    }
+    lk.unlock();
+    if (r.error) std::rethrow_exception(r.error);
 }
 
 void fields_at(const std::vector<ChargedParticle*>& particles, const std::vector<double>& times,
@@ -200,6 +286,13 @@ void deposit_only(int n_pseudo, int n_points, const double* ts, const double* fi
           tg_deposit_only(ctx, n_pseudo, n_points, ts, fields, trace->get_t0(), trace->get_dt(), (int)trace->get_N(),
                           trace->data()),
           "tg_deposit_only");
+}
+
+unsigned long long launch_count()
+{
+    State& s = state();
+    std::lock_guard<std::mutex> g(s.mu);
+    return s.launches;
 }
 
 void shutdown()

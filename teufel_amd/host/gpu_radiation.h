@@ -1,9 +1,12 @@
 // gpu_radiation.h - adapter from the host classes to the C ABI
 // (include/teufel_gpu.h).  One device context per process, created on first
-// use; calls are serialised by a mutex (the reference may call
-// integrateFieldTrace from OpenMP threads, src/mesh.cpp:296-301; the efficient
-// path is one Observer::integrate() for all pixels).  Non-zero status codes
-// of the C ABI become IOexception.
+// use.  integrate() may be called from many threads at once, as the
+// reference's per-pixel OpenMP loop calls integrateFieldTrace
+// (src/mesh.cpp:296-301): concurrent calls are combined, one caller launching
+// all requests queued so far, one batched launch per particle list, while the
+// others wait, and every call returns once its own traces hold its fields.
+// Observer::integrate() hands all pixels over in one call.  Non-zero status
+// codes of the C ABI become IOexception.
 #pragma once
 
 #include <vector>
@@ -40,6 +43,10 @@ void fields_at(const std::vector<ChargedParticle*>& particles, const std::vector
 // The device bucket deposit driven by injected retarded times ts[n_pseudo][n_points]
 // and fields fields[n_pseudo][n_points][6], accumulated into trace.
 void deposit_only(int n_pseudo, int n_points, const double* ts, const double* fields, FieldTrace* trace);
+
+// Number of batched launches integrate() has made in this process (combined
+// concurrent calls count once), for tests and reports.
+unsigned long long launch_count();
 
 // Release the device context (also done at process exit).
 void shutdown();
