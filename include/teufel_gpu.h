@@ -117,6 +117,26 @@ int tg_synchronize(tg_context* ctx);
 int tg_integrate_range(tg_context* ctx, int precision, const tg_tuning* tuning,
                        int screen_id, int p_begin, int p_end);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  As tg_integrate_range, with all launch
+ * scratch (the work-list counter, the screen descriptors, the work list and
+ * the per-wave cursor slots) in a static device array of
+ * TG_STATIC_SCRATCH_BYTES per device instead of buffers the library
+ * allocates, and the particle range as one chunk, so that the fields are added
+ * straight into the trace store (no partial traces).  It calls no hipMalloc or
+ * hipFree and no allocating library routine (the work list is planned in host
+ * memory).  Set-up comes first: the trajectory records of the precision must
+ * be packed, and the screens tiled, by an earlier tg_integrate,
+ * tg_integrate_range or tg_autotune of that precision (TG_ERR_STATE
+ * otherwise).  A launch whose scratch would exceed the array fails with
+ * TG_ERR_RANGE, naming how many particles per call fit.  All contexts on one
+ * device share that device's array, which keeps the last call's contents:
+ * calls that overlap on one GPU race.  Synchronizes the context's stream. */
+#define TG_STATIC_SCRATCH_BYTES (256u << 20)
+int tg_integrate_range_static(tg_context* ctx, int precision, const tg_tuning* tuning,
+                              int screen_id, int p_begin, int p_end);
+
 /* Copy a screen's traces to host, out[n_pix][n_bins][6]; accumulate = 1 adds
  * into out (FieldTrace += semantics, src/fieldtrace.cpp:185-190), 0 overwrites.
  * Replaces getBuffer + MPI_Reduce + fromBuffer (src/mesh.cpp:375-425,

@@ -178,6 +178,12 @@ class RadiationContext:
     def integrate_range(self, precision, p_begin, p_end, screen_id=-1, tuning=None):
         self._chk(self.lib.tg_integrate_range(self.h, precision, self._tuning(tuning), screen_id, p_begin, p_end))
 
+    def integrate_range_static(self, precision, p_begin, p_end, screen_id=-1, tuning=None):
+        """tg_integrate_range with its launch scratch in the library's static
+        device array (no device allocation; see include/teufel_gpu.h)."""
+        self._chk(self.lib.tg_integrate_range_static(self.h, precision, self._tuning(tuning), screen_id, p_begin,
+                                                     p_end))
+
     def synchronize(self):
         self._chk(self.lib.tg_synchronize(self.h))
 

@@ -38,6 +38,7 @@ SIGNATURES = [
     ("tg_integrate", _i, [_vp, _i, ctypes.POINTER(TgTuning)]),
     ("tg_synchronize", _i, [_vp]),
     ("tg_integrate_range", _i, [_vp, _i, ctypes.POINTER(TgTuning), _i, _i, _i]),
+    ("tg_integrate_range_static", _i, [_vp, _i, ctypes.POINTER(TgTuning), _i, _i, _i]),
     ("tg_read_traces", _i, [_vp, _i, _dp, _i]),
     ("tg_screen_traces_device", _i, [_vp, _i, ctypes.POINTER(_vp)]),
     ("tg_screen_poynting", _i, [_vp, _i, _dp]),

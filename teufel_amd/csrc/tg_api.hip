@@ -106,6 +106,9 @@ struct tg_context {
     // tuning
     tg_tuning tuned[2] = {};
     bool has_tuned[2] = {false, false};
+    // this device's static scratch array (tg_integrate_range_static)
+    void* static_scratch = nullptr;
+    std::string static_scratch_err;
     // graph cache
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -593,23 +596,34 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
     return TG_OK;
 }
 
-// Queue radiation kernel + chunk reductions (capturable: kernels only).
-int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
+// The buffers one launch of the radiation kernel works in: the work list,
+// the work-list counter, the screen descriptors and the per-wave cursor slots.
+struct LaunchBuffers {
+    const WorkItem* items;
+    int* next_item;
+    const ScreenDev* sdev;
+    unsigned char* slots;
+    size_t slot_bytes;
+    int lane_offsets;
+    unsigned long long* stats;
+};
+
+// Queue the radiation kernel of plan on the context's stream, the work-list
+// counter zeroed by a kernel first (no memset, so the sequence can be
+// captured); events[0..1] around the radiation kernel when events.
+int launch_radiation(tg_context* c, int prec, const LaunchPlan& plan, const LaunchBuffers& b, bool events)
 {
-    if (plan.items.empty()) return TG_OK;
-    set_int_kernel<<<1, 64, 0, c->stream>>>(c->d_next_item, 0);
+    set_int_kernel<<<1, 64, 0, c->stream>>>(b.next_item, 0);
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
     const unsigned grid = (unsigned)plan.slots;
     const int n_items = (int)plan.items.size();
-    unsigned long long* st = c->stats_on ? c->d_stats : nullptr;
-    const int lo = plan.lane_cursors ? 1 : 0;
     const RecF64* r64 = (const RecF64*)c->d_rec[0];
     const RecMixed* rmx = (const RecMixed*)c->d_rec[1];
-#define TG_LAUNCH(P, L, R)                                                                                       \
-    radiation_kernel<P, L><<<grid, kLanes, plan.lds, c->stream>>>(c->d_items, n_items, c->d_next_item, c->d_sdev, \
-                                                                  R, c->n_points, c->d_slots, plan.slot_bytes,   \
-                                                                  plan.chunk, lo, st, plan.max_window)
+#define TG_LAUNCH(P, L, R)                                                                                        \
+    radiation_kernel<P, L><<<grid, kLanes, plan.lds, c->stream>>>(b.items, n_items, b.next_item, b.sdev, R,        \
+                                                                  c->n_points, b.slots, b.slot_bytes, plan.chunk, \
+                                                                  b.lane_offsets, b.stats, plan.max_window)
     if (prec == TG_FP64 && plan.lanes)
         TG_LAUNCH(0, true, r64);
     else if (prec == TG_FP64)
@@ -621,6 +635,17 @@ int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 #undef TG_LAUNCH
     TG_HIP(c, hipGetLastError());
     if (events) TG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    return TG_OK;
+}
+
+// Queue radiation kernel + chunk reductions (capturable: kernels only).
+int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
+{
+    if (plan.items.empty()) return TG_OK;
+    const LaunchBuffers b{c->d_items, c->d_next_item, c->d_sdev, c->d_slots, plan.slot_bytes,
+                          plan.lane_cursors ? 1 : 0, c->stats_on ? c->d_stats : nullptr};
+    int rc;
+    if ((rc = launch_radiation(c, prec, plan, b, events))) return rc;
     for (size_t i = 0; i < plan.sdev.size() && plan.n_chunks > 1; ++i) {
         const ScreenHost& s = c->screens[plan.screen_ids[i]];
         size_t n = (size_t)s.n_pix * s.n_bins * 6;
@@ -634,26 +659,28 @@ int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
 
 int check_ctx(const tg_context* c) { return c ? TG_OK : TG_ERR_ARG; }
 
-int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id, int p0, int p1)
+// Argument checks of an integrate call; fills ids with the screens that take
+// part (empty: a silent no-op of the reference, nots < 2 or NP < 2,
+// particle.cpp:778,783).
+int integrate_args(tg_context* c, int prec, int screen_id, int p0, int p1, std::vector<int>& ids)
 {
+    ids.clear();
     if (prec != TG_FP64 && prec != TG_MIXED) return fail(c, TG_ERR_ARG, "precision must be TG_FP64 or TG_MIXED");
     if (!c->traj_set && !c->screens.empty()) return fail(c, TG_ERR_STATE, "tg_integrate: no trajectories set");
     if (screen_id >= (int)c->screens.size()) return fail(c, TG_ERR_RANGE, "screen id out of range");
     if (p0 < 0 || p1 > c->n_particles || p0 > p1) return fail(c, TG_ERR_RANGE, "particle range out of range");
     c->kernel_ms = c->total_ms = 0.f;
-    // silent no-ops of the reference: nots < 2, NP < 2 (particle.cpp:778,783)
     if (c->n_points < 2 || p0 == p1) return TG_OK;
-    std::vector<int> ids;
     for (int i = 0; i < (int)c->screens.size(); ++i)
         if ((screen_id < 0 || i == screen_id) && c->screens[i].n_bins >= 2 && c->screens[i].n_pix > 0)
             ids.push_back(i);
-    if (ids.empty()) return TG_OK;
-    int rc;
-    if ((rc = ensure_records(c, prec))) return rc;
-    for (int id : ids)
-        if ((rc = ensure_tiles(c, c->screens[id]))) return rc;
-    // the autotuned shape, with every non-zero field of the caller's tuning
-    // taking precedence
+    return TG_OK;
+}
+
+// The autotuned shape, with every non-zero field of the caller's tuning
+// taking precedence.
+tg_tuning effective_tuning(const tg_context* c, int prec, const tg_tuning* tun)
+{
     tg_tuning eff = c->has_tuned[prec] ? c->tuned[prec] : tg_tuning{};
     if (tun) {
         if (tun->bins_per_window > 0) eff.bins_per_window = tun->bins_per_window;
@@ -663,6 +690,19 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         if (tun->spill > 0) eff.spill = tun->spill;
         eff.use_graph = tun->use_graph;
     }
+    return eff;
+}
+
+int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id, int p0, int p1)
+{
+    std::vector<int> ids;
+    int rc;
+    if ((rc = integrate_args(c, prec, screen_id, p0, p1, ids))) return rc;
+    if (ids.empty()) return TG_OK;
+    if ((rc = ensure_records(c, prec))) return rc;
+    for (int id : ids)
+        if ((rc = ensure_tiles(c, c->screens[id]))) return rc;
+    tg_tuning eff = effective_tuning(c, prec, tun);
     bool use_graph = eff.use_graph != 0;
     char keybuf[256];
     snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window,
@@ -719,6 +759,111 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         }
         c->last_items = (long long)plan.items.size();
     }
+    return TG_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch scratch without allocation (tg_integrate_range_static).  All of that
+// entry point's launch scratch (the work-list counter, the screen
+// descriptors, the work list and the per-wave cursor slots) lives in one
+// file-scope device array, one copy per device, held until the process ends.
+// Host code never hands the array's name to a kernel or a copy: its device
+// address on each device comes from hipGetSymbolAddress (static_scratch_of).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[TG_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;      // device address of `scratch`, per device
+std::vector<std::string> g_scratch_err;
+
+// This device's copy of the array, looked up once per device with the device
+// current (tg_create's set-up); nullptr with the reason in err if the lookup
+// failed.
+void* static_scratch_of(int device, std::string& err)
+{
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) {
+        g_scratch_ptr.resize(device + 1, nullptr);
+        g_scratch_err.resize(device + 1);
+    }
+    if (!g_scratch_ptr[device] && g_scratch_err[device].empty()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        } else {
+            g_scratch_ptr[device] = p;
+        }
+    }
+    err = g_scratch_err[device];
+    return g_scratch_ptr[device];
+}
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// tg_integrate_range_static: one particle chunk, all scratch in the array.
+int integrate_static(tg_context* c, int prec, const tg_tuning* tun, int screen_id, int p0, int p1)
+{
+    const char* who = "tg_integrate_range_static: ";
+    std::vector<int> ids;
+    int rc;
+    if ((rc = integrate_args(c, prec, screen_id, p0, p1, ids))) return rc;
+    if (ids.empty()) return TG_OK;
+    if (!c->static_scratch) return fail(c, TG_ERR_HIP, who + c->static_scratch_err);
+    // set-up allocates; this entry point only uses what set-up made
+    if (!c->rec_valid[prec])
+        return fail(c, TG_ERR_STATE,
+                    std::string(who) + "the trajectory records of this precision are not packed yet (the first "
+                                       "tg_integrate, tg_integrate_range or tg_autotune of the precision packs them)");
+    for (int id : ids)
+        if (!c->screens[id].perm_valid)
+            return fail(c, TG_ERR_STATE,
+                        std::string(who) + "screen " + std::to_string(id) +
+                            " is not tiled yet (the first tg_integrate, tg_integrate_range or tg_autotune tiles it)");
+    tg_tuning eff = effective_tuning(c, prec, tun);
+    eff.particles_per_chunk = p1 - p0;   // one chunk: fields straight into the trace store
+    eff.use_graph = 0;
+    LaunchPlan plan;
+    if ((rc = make_plan(c, prec, ids, p0, p1, &eff, plan))) return rc;
+    if (plan.items.empty()) return TG_OK;
+    if (plan.n_chunks != 1) return fail(c, TG_ERR_STATE, std::string(who) + "plan with more than one chunk");
+    // layout: counter | screen descriptors | work list | cursor slots
+    const size_t off_sdev = 256;
+    const size_t off_items = align256(off_sdev + plan.sdev.size() * sizeof(ScreenDev));
+    const size_t off_slots = align256(off_items + plan.items.size() * sizeof(WorkItem));
+    auto slot_size = [&](bool offsets) {
+        return align256((size_t)plan.chunk * sizeof(int2) + (offsets ? (size_t)plan.chunk * kLanes : 0));
+    };
+    bool offsets = plan.lanes && c->lane_cursors_on;
+    if (off_slots + (size_t)plan.slots * slot_size(offsets) > (size_t)TG_STATIC_SCRATCH_BYTES) offsets = false;
+    const size_t need = off_slots + (size_t)plan.slots * slot_size(offsets);
+    if (need > (size_t)TG_STATIC_SCRATCH_BYTES) {
+        const size_t room = (size_t)TG_STATIC_SCRATCH_BYTES > off_slots ? (size_t)TG_STATIC_SCRATCH_BYTES - off_slots : 0;
+        const long long fit = (long long)(room / std::max(1, plan.slots) / sizeof(int2));
+        return fail(c, TG_ERR_RANGE,
+                    std::string(who) + "the launch needs " + std::to_string(need) + " bytes of scratch, the static " +
+                        "array holds " + std::to_string((size_t)TG_STATIC_SCRATCH_BYTES) + "; integrate at most about " +
+                        std::to_string(std::max(0LL, fit - 8)) + " particles per call");
+    }
+    unsigned char* base = static_cast<unsigned char*>(c->static_scratch);
+    for (ScreenDev& d : plan.sdev) d.partial = nullptr;   // chunk 0 only: no partial traces
+    TG_HIP(c, hipMemcpyAsync(base + off_sdev, plan.sdev.data(), plan.sdev.size() * sizeof(ScreenDev),
+                             hipMemcpyHostToDevice, c->stream));
+    TG_HIP(c, hipMemcpyAsync(base + off_items, plan.items.data(), plan.items.size() * sizeof(WorkItem),
+                             hipMemcpyHostToDevice, c->stream));
+    const LaunchBuffers b{reinterpret_cast<const WorkItem*>(base + off_items), reinterpret_cast<int*>(base),
+                          reinterpret_cast<const ScreenDev*>(base + off_sdev), base + off_slots, slot_size(offsets),
+                          offsets ? 1 : 0, nullptr};
+    if ((rc = launch_radiation(c, prec, plan, b, true))) return rc;
+    TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    TG_HIP(c, hipEventSynchronize(c->ev[2]));   // the host copies' sources live until here
+    TG_HIP(c, hipEventElapsedTime(&c->kernel_ms, c->ev[0], c->ev[1]));
+    TG_HIP(c, hipEventElapsedTime(&c->total_ms, c->ev[0], c->ev[2]));
     return TG_OK;
 }
 
@@ -789,6 +934,7 @@ int tg_create(int device, tg_context** out)
     for (const void* k : kernels)
         if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->max_lds) != hipSuccess)
             (void)hipGetLastError();
+    c->static_scratch = static_scratch_of(device, c->static_scratch_err);
     *out = c;
     return TG_OK;
 }
@@ -960,6 +1106,14 @@ int tg_integrate_range(tg_context* c, int precision, const tg_tuning* tuning, in
     if (check_ctx(c)) return TG_ERR_ARG;
     TG_HIP(c, hipSetDevice(c->device));
     return integrate_impl(c, precision, tuning, screen_id, p_begin, p_end);
+}
+
+int tg_integrate_range_static(tg_context* c, int precision, const tg_tuning* tuning, int screen_id, int p_begin,
+                              int p_end)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    TG_HIP(c, hipSetDevice(c->device));
+    return integrate_static(c, precision, tuning, screen_id, p_begin, p_end);
 }
 
 int tg_synchronize(tg_context* c)
