@@ -34,14 +34,16 @@ namespace tg {
 // Packing: raw ChargedParticle trajectory (t, X, u=beta*gamma, a) -> records.
 // ---------------------------------------------------------------------------
 
-// Record of point k of particle p into the blocked layout (rec_offset).
+// Record of point k of particle p into the piece-major layout (rec_piece_offset).
 template <class Rec>
 __device__ __forceinline__ void store_rec(char* out, int n_points, size_t p, size_t k, const Rec& r)
 {
-    char* dst = out + p * rec_particle_bytes<Rec>(n_points) + rec_offset<Rec>((int)k);
+    char* dst = out + p * rec_particle_bytes<Rec>(n_points);
+    const unsigned stride = rec_piece_stride(n_points);
     const uint4* src = reinterpret_cast<const uint4*>(&r);
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(Rec) / 16); ++i) *reinterpret_cast<uint4*>(dst + kRecPieceStride<Rec> * i) = src[i];
+    for (int i = 0; i < kRecPieces<Rec>; ++i)
+        *reinterpret_cast<uint4*>(dst + rec_piece_offset((int)k, i, stride)) = src[i];
 }
 
 __global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict__ t,
@@ -169,7 +171,7 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
     }
     double mx = nx - r.bx, my = ny - r.by, mz = nz - r.bz;   // n - beta
     double k3 = kappa * kappa * kappa;
-    double irk3 = rcp_rn(R * k3);                             // 1/(R kappa^3)
+    double irk3 = rcp_amp(R * k3);                            // 1/(R kappa^3)
     double cv = r.g2inv * invR * irk3;                        // 1/(R^2 kappa^3 gamma^2)
     double ca = irk3 * kInvSpeedOfLight;                      // 1/(R kappa^3 c)
     double nbp = nx * r.bpx + ny * r.bpy + nz * r.bpz;        // n . beta'
@@ -204,8 +206,9 @@ __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, dou
     const double e = __builtin_fma(-h, y, 1.0);
     y = __builtin_fma(0.5 * y, e, y);
     const double R = r2 * y;
-    const double ts = __builtin_fma(R, kInvSpeedOfLight, t);
-    const double q = (ts - t0) * inv_dt;
+    // q = (t + R/c - t0)/dt as R (1/(c dt)) + (t/dt - t0/dt): both terms and
+    // q keep ~1e-9 of a bucket (t/dt ~ 1e7)
+    const double q = __builtin_fma(R, kInvSpeedOfLight * inv_dt, __builtin_fma(t, inv_dt, -(t0 * inv_dt)));
     const double r = __builtin_fmin(__builtin_fmax(__builtin_rint(q), -1.0e9), 1.0e9);
     idx = (int)r;
     u = (float)(q - r);
@@ -264,6 +267,9 @@ __device__ __forceinline__ void eval_point(const typename RecOf<PREC>::type& r, 
 // Reciprocals of the deposit weights (amplitude-level, not timing).
 struct RcpRn {
     __device__ __forceinline__ double operator()(double x) const { return rcp_rn(x); }
+};
+struct RcpAmp {
+    __device__ __forceinline__ double operator()(double x) const { return rcp_amp(x); }
 };
 struct RcpF32 {
     __device__ __forceinline__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
@@ -351,9 +357,11 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 // per-lane walk are bound by the vector-memory address path, which handles
 // every load instruction separately, and a field-typed copy of the record
 // compiles to more, narrower loads.  The resource covers the particle's
-// records (in the blocked layout, rec_offset): a read past them (the walk's
-// read-ahead near the end of a particle) returns zeros, which only stopped
-// lanes evaluate.
+// records (piece-major, rec_piece_offset): the point's offset 16 k is the
+// per-lane offset, the piece's offset the wave-uniform scalar offset.  The
+// walk's read-ahead near the end of a particle reads the zero padding of each
+// piece (or, past it, zeros from the resource), which only stopped lanes
+// evaluate.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kBufferWord3 = 0x00020000;   // raw 32-bit data format, gfx9 buffer resource word 3
 
@@ -366,14 +374,16 @@ __device__ __forceinline__ float flt(unsigned v) { return __builtin_bit_cast(flo
 template <class Rec>
 struct RecBuffer {
     __amdgpu_buffer_rsrc_t rsrc;
+    int stride;   // bytes between the pieces of a point (wave-uniform)
     __device__ __forceinline__ RecBuffer(const char* rp, int NP)
         : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(rp), (short)0, (int)rec_particle_bytes<Rec>(NP),
-                                                 kBufferWord3))
+                                                 kBufferWord3)),
+          stride((int)rec_piece_stride(NP))
     {
     }
     __device__ __forceinline__ u32x4 part(int k, int i) const
     {
-        return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(rec_offset<Rec>(k) + kRecPieceStride<Rec> * i), 0, 0);
+        return __builtin_amdgcn_raw_buffer_load_b128(rsrc, 16 * k, stride * i, 0);
     }
     __device__ __forceinline__ Rec load(int k) const;
 };
@@ -416,15 +426,10 @@ __device__ __forceinline__ RecF64 RecBuffer<RecF64>::load(int k) const
     return r;
 }
 
-__device__ __forceinline__ int wave_min(int v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int o = __shfl_xor(v, off);
-        v = o < v ? o : v;
-    }
-    return v;
-}
+// The minimum over the wave's active lanes (all lanes where it is called),
+// by the device library's cross-lane reduction (DPP row operations and a
+// readlane: no LDS round trips).
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
 
 // Deposit of step (p1 -> p2) into the window [b0, b1), FP64: retarded times
 // and bucket edges as the reference forms them.  With spill, the window has
@@ -442,7 +447,7 @@ __device__ __forceinline__ void deposit_window(const Window<double>& win, bool a
     const bool case_a = p1.ts <= tmin;
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
-    const TwoBuckets<double> t = step_common<double>(RcpRn{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
+    const TwoBuckets<double> t = step_common<double>(RcpAmp{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
                                                      inv_dt, n_bins, b0, b1 + (spill ? 1 : 0), -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
@@ -492,21 +497,29 @@ constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
 
 template <class Rec>
 struct LdsStager {
-    const char* __restrict__ rp;      // the particle's records (blocked layout; a chunk is 2 whole blocks)
-    double* slots;                    // LDS: 2 chunks
+    const char* __restrict__ rp;      // the particle's records (piece-major)
+    double* slots;                    // LDS: 2 chunks, each [piece][kChunkRecs points][2 doubles]
     int lane;
-    int lim;                          // last legal 3-double group start in the particle's records
+    int last_chunk;                   // last chunk of kChunkRecs points within each piece
+    unsigned stride;                  // bytes between pieces
     int next_fill;                    // chunk held in (q0,q1,q2)
     double q0, q1, q2;
 
+    static constexpr int kPieceDoubles = 2 * kChunkRecs;   // one piece of a chunk: 32 doubles
+    // the three doubles lane `lane` loads of chunk `chunk`: element g of the
+    // chunk's [piece][point][2] doubles, g = 3 lane + e
+    __device__ __forceinline__ double elem(int chunk, int e) const
+    {
+        const int g = 3 * lane + e;
+        const int piece = g / kPieceDoubles, w = g % kPieceDoubles;
+        const double* base = reinterpret_cast<const double*>(rp + (size_t)piece * stride);
+        return base[kPieceDoubles * (chunk < last_chunk ? chunk : last_chunk) + w];
+    }
     __device__ __forceinline__ void load(int chunk, double& a, double& b, double& c) const
     {
-        const double* base = reinterpret_cast<const double*>(rp);
-        int i = kChunkDoubles * chunk + 3 * lane;
-        i = i < lim ? i : lim;
-        a = base[i];
-        b = base[i + 1];
-        c = base[i + 2];
+        a = elem(chunk, 0);
+        b = elem(chunk, 1);
+        c = elem(chunk, 2);
     }
     __device__ __forceinline__ void store(int chunk, double a, double b, double c) const
     {
@@ -528,23 +541,22 @@ struct LdsStager {
         store(c0 + 1, d, e, f);
         next_fill = c0 + 2;
     }
-    // Record r of a staged chunk, gathered from its pieces (blocked layout).
+    // Record r of a staged chunk, gathered from its pieces.
     __device__ __forceinline__ Rec at(int r) const
     {
-        static_assert(kChunkRecs % kRecBlock<Rec> == 0, "a chunk is whole blocks");
-        constexpr int rd = (int)(sizeof(Rec) / sizeof(double));
+        static_assert(kPieces * kPieceDoubles == kChunkDoubles, "a chunk is the pieces of kChunkRecs points");
         const int j = r % kChunkRecs;
-        const double* b = slots + ((r / kChunkRecs) & 1) * kChunkDoubles + (j / kRecBlock<Rec>) * (kRecBlock<Rec> * rd) +
-                          (j % kRecBlock<Rec>) * 2;
+        const double* b = slots + ((r / kChunkRecs) & 1) * kChunkDoubles + 2 * j;
         Rec out;
         double* o = reinterpret_cast<double*>(&out);
 #pragma unroll
-        for (int i = 0; i < rd / 2; ++i) {
-            o[2 * i] = b[(kRecPieceStride<Rec> / sizeof(double)) * i];
-            o[2 * i + 1] = b[(kRecPieceStride<Rec> / sizeof(double)) * i + 1];
+        for (int i = 0; i < kPieces; ++i) {
+            o[2 * i] = b[kPieceDoubles * i];
+            o[2 * i + 1] = b[kPieceDoubles * i + 1];
         }
         return out;
     }
+    static constexpr int kPieces = kRecPieces<Rec>;
     // Record r, r advancing by one per call after start.
     __device__ __forceinline__ Rec next(int r)
     {
@@ -560,7 +572,7 @@ struct LdsStager {
 
 template <class Rec>
 struct ScalarPrefetch {
-    const char* __restrict__ rp;   // the particle's records (blocked layout)
+    const char* __restrict__ rp;   // the particle's records (piece-major)
     int NP;
     Rec rn;   // record of the next point
     // record k (uniform), gathered from its pieces
@@ -568,9 +580,10 @@ struct ScalarPrefetch {
     {
         Rec r;
         uint4* o = reinterpret_cast<uint4*>(&r);
-        const char* b = rp + rec_offset<Rec>(k);
+        const unsigned stride = rec_piece_stride(NP);
 #pragma unroll
-        for (int i = 0; i < (int)(sizeof(Rec) / 16); ++i) o[i] = *reinterpret_cast<const uint4*>(b + kRecPieceStride<Rec> * i);
+        for (int i = 0; i < kRecPieces<Rec>; ++i)
+            o[i] = *reinterpret_cast<const uint4*>(rp + rec_piece_offset(k, i, stride));
         return r;
     }
     __device__ __forceinline__ void start(int k) { rn = at(k + 1); }
@@ -594,8 +607,8 @@ template <int PREC>
 __device__ __forceinline__ typename SourceOf<PREC>::type make_source(const char* rp, int NP, double* stage, int lane)
 {
     if constexpr (PREC == 0)
-        return LdsStager<RecF64>{rp, stage, lane, (int)(rec_particle_bytes<RecF64>(NP) / sizeof(double)) - 3, 0, 0.0,
-                                 0.0, 0.0};
+        return LdsStager<RecF64>{rp, stage, lane, (int)(rec_piece_stride(NP) / (16 * kChunkRecs)) - 1,
+                                 rec_piece_stride(NP), 0, 0.0, 0.0, 0.0};
     else
         return ScalarPrefetch<RecMixed>{rp, NP, RecMixed{}};
 }
@@ -828,8 +841,9 @@ __device__ __forceinline__ int search_first_point(const char* __restrict__ rp_la
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         // t, x, y, z: pieces 0 and 1 of both record types
-        const char* p = rp_lane + rec_offset<Rec>(mid);
-        const uint4 a = *reinterpret_cast<const uint4*>(p), c = *reinterpret_cast<const uint4*>(p + kRecPieceStride<Rec>);
+        const unsigned stride = rec_piece_stride(NP);
+        const uint4 a = *reinterpret_cast<const uint4*>(rp_lane + rec_piece_offset(mid, 0, stride)),
+                    c = *reinterpret_cast<const uint4*>(rp_lane + rec_piece_offset(mid, 1, stride));
         const double t = dbl(a.x, a.y), x = dbl(a.z, a.w), y = dbl(c.x, c.y), z = dbl(c.z, c.w);
         double R, ts;
         int idx;
@@ -923,7 +937,8 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         bool b0_stored = carry && carry_stored;   // out of bucket 0 already holds part of it
         bool spill_stored = false;                // out of the spill bucket (bucket nw) holds part of it
         bool dirty = carry;    // the window holds terms not yet in the partial trace
-        unsigned terms = carry ? carry_terms : 0;  // estimated terms per cell since the last flush
+        // estimated terms per cell since the last flush, times nw
+        unsigned terms = carry ? carry_terms : 0;
         // cell j of the window as an FP64 trace value (mixed cells hold c B
         // for the magnetic components)
         auto cell = [&](int j, int c) {
@@ -975,9 +990,11 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
                 if constexpr (PREC == 1) {
+                    // terms counted in units of 1/nw: a walk of ws wave-steps
+                    // adds about ws/nw + 1 terms to each of the nw cells
                     dirty = true;
-                    terms += (unsigned)ws / (unsigned)nw + 1;
-                    if (terms >= kFlushTerms) flush();
+                    terms += (unsigned)ws + (unsigned)nw;
+                    if (terms >= kFlushTerms * (unsigned)nw) flush();
                 }
             }
             if (b1 < it.bin_hi && lane < npb) cur_base[pb - it.p0 + lane] = ncur;

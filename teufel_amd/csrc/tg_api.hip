@@ -201,8 +201,8 @@ int ensure_records(tg_context* c, int prec)
     if (c->rec_valid[prec]) return TG_OK;
     size_t n = (size_t)c->n_particles * c->n_points;
     if (!c->d_rec[prec]) {
-        // blocked layout (rec_offset): whole blocks per particle, the padding
-        // points zero (the pack kernels write only real points)
+        // piece-major layout (rec_piece_offset): the padding points of every
+        // piece zero (the pack kernels write only real points)
         const size_t bytes = std::max<size_t>(1, (size_t)c->n_particles * (prec == TG_FP64
                                                                                ? rec_particle_bytes<RecF64>(c->n_points)
                                                                                : rec_particle_bytes<RecMixed>(c->n_points)));

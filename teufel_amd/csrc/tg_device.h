@@ -58,34 +58,32 @@ struct alignas(64) RecMixed {
 static_assert(sizeof(RecF64) == 96, "RecF64 layout");
 static_assert(sizeof(RecMixed) == 64, "RecMixed layout");
 
-// Layout of the packed records in HBM.  Each particle's points are grouped
-// in blocks of kRecBlock<Rec> points; within a block the records are stored
-// piece-major: piece i (bytes [16 i, 16 i + 16) of the record) of the
-// block's points lie together.  The per-lane walk reads one piece of a
-// different point in every lane, so with blocks of 8 lanes at nearby points
-// share cache lines (8 points per 128-B line instead of 2).  That pays for
-// the mixed record (5 % on c2); for the FP64 record the extra address
-// arithmetic cost more than it saved (2 %), so its blocks are single points,
-// the plain array of records.  A particle's records take
-// rec_particle_bytes(NP) bytes, NP rounded up to whole blocks (the rest zero).
+// Layout of the packed records in HBM: piece-major per particle.  Piece i
+// (bytes [16 i, 16 i + 16) of the record) of all points of a particle lies
+// together, 16 B per point, so piece i of point k sits at
+// i * rec_piece_stride(NP) + 16 k of the particle's records.  The per-lane
+// walk reads one piece of a different point in every lane: lanes at nearby
+// points share cache lines (8 points per 128-B line, against 2 for an array
+// of mixed records and 1.3 of FP64 ones), and the address of a point is one
+// shift, with the piece offsets wave-uniform (the buffer loads' scalar
+// offset).  Each piece holds NP + 2 points rounded up to 16 (the walk reads
+// up to two points ahead; the padding is zero), so pieces start on 256-B
+// boundaries.
 template <class Rec>
-constexpr int kRecBlock = 1;
-template <>
-constexpr int kRecBlock<RecMixed> = 8;
+constexpr int kRecPieces = (int)(sizeof(Rec) / 16);
+__host__ __device__ constexpr unsigned rec_piece_stride(int NP)
+{
+    return (unsigned)((NP + 2 + 15) / 16) * 256u;
+}
 template <class Rec>
 __host__ __device__ constexpr size_t rec_particle_bytes(int NP)
 {
-    return (size_t)((NP + kRecBlock<Rec> - 1) / kRecBlock<Rec>) * kRecBlock<Rec> * sizeof(Rec);
+    return (size_t)kRecPieces<Rec> * rec_piece_stride(NP);
 }
-// Byte offset of piece 0 of point k (k >= 0) in its particle's records;
-// piece i lies kRecPieceStride<Rec> i bytes further.
-template <class Rec>
-constexpr unsigned kRecPieceStride = 16u * kRecBlock<Rec>;
-template <class Rec>
-__host__ __device__ constexpr unsigned rec_offset(int k)
+// Byte offset of piece i of point k in its particle's records.
+__host__ __device__ constexpr unsigned rec_piece_offset(int k, int i, unsigned stride)
 {
-    return ((unsigned)k / kRecBlock<Rec>) * (unsigned)(kRecBlock<Rec> * sizeof(Rec)) +
-           ((unsigned)k % kRecBlock<Rec>) * 16u;
+    return (unsigned)i * stride + 16u * (unsigned)k;
 }
 
 // One screen as the kernel sees it.  Particle chunk 0 of a launch adds its
@@ -134,6 +132,17 @@ __device__ __forceinline__ double rcp_rn(double b)
     y = __builtin_fma(y, e, y);
     e = __builtin_fma(-b, y, 1.0);
     return __builtin_fma(y, e, y);
+}
+
+// 1/b to ~2^-46 relative (the hardware estimate and one Newton step with
+// FMA), for reciprocals that scale amplitudes only (the deposit weights, the
+// 1/(R kappa^3) of the field), where the trace tolerance (1e-12 of each
+// pixel's largest field) leaves ~1e4 of room.  Timing and n = R/|R|, whose
+// rounding must match the reference's, use rcp_rn.
+__device__ __forceinline__ double rcp_amp(double b)
+{
+    const double y = __builtin_amdgcn_rcp(b);
+    return __builtin_fma(y, __builtin_fma(-b, y, 1.0), y);
 }
 
 // RN(sqrt(x)) for normal x > 0 far from the denormal range: Goldschmidt
