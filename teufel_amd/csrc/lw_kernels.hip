@@ -483,6 +483,65 @@ __device__ __forceinline__ void deposit_window(const Window<float>& win, bool ac
     }
 }
 
+// Window cell adder of the lane accumulator (lane_step, lane_close).
+template <class F>
+struct WindowAdd {
+    Window<F> w;
+    __device__ __forceinline__ void operator()(int j, int c, F v) const { Window<F>::add(w.cell(j) + c * kLanes, v); }
+};
+
+// The per-lane walk's deposit of step (p1 -> p2): the rule of
+// deposit_window (same weights, spill and general steps), with the common
+// steps summed per bucket in the lane accumulator L (lane_step) instead of
+// added to the window step by step.  The accumulator of a lane that does not
+// walk the step takes garbage (it is closed when the lane stops, and the
+// next walk starts it afresh), so its update needs no mask.  FP64.
+__device__ __forceinline__ void deposit_lane(const Window<double>& win, LaneAcc<double>& L, bool walking,
+                                             const Point& p1, const Point& p2, double t0, double dt, double inv_dt,
+                                             double tmin, double tmax, int n_bins, int b0, int b1, bool spill)
+{
+    const bool valid = walking && p2.ts > tmin && p1.ts < tmax;    // particle.cpp:793
+    const bool case_a = p1.ts <= tmin;
+    const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
+    const bool gen = valid && step_is_general(case_a, p1.idx, i2);
+    const bool common = valid && !gen;
+    const TwoBuckets<double> t = step_common<double>(RcpAmp{}, common, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt,
+                                                     n_bins, b0, b1, -1, -1);
+    const int ja = (unsigned)(p1.idx - b0) < (unsigned)(b1 - b0) ? p1.idx - b0 : -1;
+    lane_step(L, WindowAdd<double>{win}, common, walking && (gen || i2 != p1.idx), t, ja, p1.f, p2.f);
+    if (__ballot(gen)) {
+        if (gen) {
+            WindowAcc<double> acc{win, b0, p1.f, p2.f};
+            step_general(acc, rcp_rn(p2.ts - p1.ts), case_a, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt, n_bins,
+                         b0, b1 + ((spill && i2 == b1) ? 1 : 0));
+        }
+    }
+}
+
+// Mixed: the same in bucket units (deposit_window for the mixed kernel).
+__device__ __forceinline__ void deposit_lane(const Window<float>& win, LaneAcc<float>& L, bool walking,
+                                             const PointM& p1, const PointM& p2, double, double, double, double,
+                                             double, int n_bins, int b0, int b1, bool spill)
+{
+    const bool valid = walking && p2.idx >= 0 && p1.idx < n_bins;  // particle.cpp:793
+    const bool case_a = p1.idx < 0;                               // particle.cpp:796
+    const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
+    const bool gen = valid && step_is_general(case_a, p1.idx, i2);
+    const bool common = valid && !gen;
+    const TwoBuckets<float> t = step_common_units<float>(RcpF32{}, common, p1.idx, p1.u, p2.idx, p2.u, i2, b0, b1,
+                                                         -1, -1);
+    const int ja = (unsigned)(p1.idx - b0) < (unsigned)(b1 - b0) ? p1.idx - b0 : -1;
+    lane_step(L, WindowAdd<float>{win}, common, walking && (gen || i2 != p1.idx), t, ja, p1.f, p2.f);
+    if (__ballot(gen)) {
+        if (gen) {
+            WindowAcc<float> acc{win, b0, p1.f, p2.f};
+            const double q1 = (double)p1.idx + (double)p1.u, q2 = (double)p2.idx + (double)p2.u;
+            step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0,
+                         b1 + ((spill && i2 == b1) ? 1 : 0));
+        }
+    }
+}
+
 // Two ways to feed the wave the (uniform) trajectory record of each step.
 //
 // LdsStager (FP64 kernel, 96-B records): the particle's records are staged
@@ -787,24 +846,37 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
     // into the current one (on c2 6 % faster for FP64, 1 % for mixed).
     Pt pb;
     Rec rb;
+    // The lane's bucket sum (deposit_lane), added to the window as the lane
+    // stops after a step that took it into bucket idx0: the spill bucket, or
+    // the bucket of the particle's last point.
+    using F = typename RecOf<PREC>::amp;
+    LaneAcc<F> L;
+    lane_begin(L);
+    const WindowAdd<F> wadd{win};
     auto step = [&](const Rec& rc, Rec& rnext, const Pt& ps, Pt& pe) {
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         __builtin_amdgcn_sched_barrier(0);
         rnext = rbuf.load(kl + 2);
         eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
-        deposit_window(win, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, spill);
+        deposit_lane(win, L, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, spill);
         ++steps;
         if (walking) {
             if (pe.idx >= b1) {
                 walking = false;
-                if (spill && pe.idx == b1) {   // deposited whole, into the spill bucket
+                if (spill && pe.idx == b1) {   // deposited whole, the rest of it into the spill bucket
                     ++kl;
                     idx0 = pe.idx;
+                    lane_fold(L, pe.f);
+                    lane_close(L, wadd, b1 - b0);
                 }
             } else {
                 ++kl;
                 idx0 = pe.idx;
                 walking = kl < NP - 1;
+                if (!walking) {   // the particle's last point
+                    lane_fold(L, pe.f);
+                    if ((unsigned)(idx0 - b0) < (unsigned)(b1 - b0)) lane_close(L, wadd, idx0 - b0);
+                }
             }
         }
     };
@@ -836,7 +908,6 @@ template <int PREC>
 __device__ __forceinline__ int search_first_point(const char* __restrict__ rp_lane, int NP, double Px, double Py,
                                                   double Pz, double t0, double dt, double inv_dt, int b)
 {
-    using Rec = typename RecOf<PREC>::type;
     int lo = 0, hi = NP - 1;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -867,9 +938,12 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
 {
     using F = typename RecOf<PREC>::amp;
     // Mixed: the FP32 window is flushed into the FP64 partial trace once its
-    // cells may hold about kFlushTerms summed terms (a walk of T wave-steps
-    // adds about T/W terms to each of its W cells, counted as T/W + 1), so an
-    // FP32 sum never runs much beyond 2048 terms (~2^-24 sqrt(2048) relative).
+    // cells may hold about kFlushTerms summed terms, so an FP32 sum never
+    // runs much beyond 2048 terms (~2^-24 sqrt(2048) relative).  A per-lane
+    // walk adds a lane's bucket sum once to each cell it reaches, and a
+    // general step adds to a cell directly, counted as 3 terms per walk; a
+    // wave-uniform walk of T wave-steps adds about T/W terms to each of its W
+    // cells, counted as T/W + 1.
     constexpr unsigned kFlushTerms = 2048;
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
@@ -990,10 +1064,9 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                 ++n_walks;
                 if (lane == j) ncur = make_int2(nk, nmin);
                 if constexpr (PREC == 1) {
-                    // terms counted in units of 1/nw: a walk of ws wave-steps
-                    // adds about ws/nw + 1 terms to each of the nw cells
+                    // terms counted in units of 1/nw
                     dirty = true;
-                    terms += (unsigned)ws + (unsigned)nw;
+                    terms += LANES ? 3u * (unsigned)nw : (unsigned)ws + (unsigned)nw;
                     if (terms >= kFlushTerms * (unsigned)nw) flush();
                 }
             }

@@ -105,6 +105,76 @@ TG_HD inline TwoBuckets<F> step_common_units(const Rcp& rcp, bool common, int id
     return o;
 }
 
+// a*b + c with one rounding, in F.
+TG_HD inline double fma_f(double a, double b, double c) { return __builtin_fma(a, b, c); }
+TG_HD inline float fma_f(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Point-weight accumulation of one lane's walk over consecutive common
+// steps (the per-lane walk of the radiation kernel).  A walk's steps arrive
+// in order and their buckets never decrease, so the lane keeps its sum for
+// the bucket of its current step's start point in acc and adds it to the
+// window once, when it leaves that bucket.  A common step adds aa f1 + ba f2
+// to its bucket idx1 and, crossing, ab f1 + bb f2 to idx1 + 1 (step_common);
+// the share of f2 in the bucket the next step starts in (ba, or bb after a
+// crossing) is held back in wc and added with the next step's own share of
+// that point, as (wc + aa) f1.  So a point's field is multiplied once per
+// bucket its two steps give it, and a bucket receives one add per lane and
+// particle: 6 products a step, 12 more and one window add per bucket
+// crossed.  The bucket sums are those of deposit_step, grouped by point.
+template <class F>
+struct LaneAcc {
+    F acc[6];
+    F wc;
+};
+
+template <class F>
+TG_HD inline void lane_begin(LaneAcc<F>& L)
+{
+    for (int c = 0; c < 6; ++c) L.acc[c] = (F)0;
+    L.wc = (F)0;
+}
+
+// Step p1 -> p2 of a lane that walks it.  common: valid and not general,
+// with weights t; any other step adds nothing here (a general one goes to
+// step_general), and its end point starts the next bucket sum from zero.
+// leaves: the step does not end in bucket idx1 (crossing or general); ja: the
+// window-relative index of bucket idx1, negative outside the window.
+// add(j, c, v) adds v to component c of window bucket j.
+template <class F, class Add>
+TG_HD inline void lane_step(LaneAcc<F>& L, const Add& add, bool common, bool leaves, const TwoBuckets<F>& t, int ja,
+                            const F* f1, const F* f2)
+{
+    const F aa = common ? t.aa : (F)0, ba = common ? t.ba : (F)0;
+    const F wa = L.wc + aa;
+    for (int c = 0; c < 6; ++c) L.acc[c] = fma_f(wa, f1[c], L.acc[c]);
+    if (leaves) {
+        if (ja >= 0)
+            for (int c = 0; c < 6; ++c) add(ja, c, fma_f(ba, f2[c], L.acc[c]));
+        const F ab = common ? t.ab : (F)0;
+        for (int c = 0; c < 6; ++c) L.acc[c] = ab * f1[c];
+        L.wc = common ? t.bb : (F)0;
+    } else {
+        L.wc = ba;
+    }
+}
+
+// The lane stops past the end point p2 of its last step: the held-back share
+// of p2 goes into acc.
+template <class F>
+TG_HD inline void lane_fold(LaneAcc<F>& L, const F* f2)
+{
+    for (int c = 0; c < 6; ++c) L.acc[c] = fma_f(L.wc, f2[c], L.acc[c]);
+    L.wc = (F)0;
+}
+
+// acc into window bucket j, the bucket of the lane's last point (after
+// lane_fold).
+template <class F, class Add>
+TG_HD inline void lane_close(const LaneAcc<F>& L, const Add& add, int j)
+{
+    for (int c = 0; c < 6; ++c) add(j, c, L.acc[c]);
+}
+
 // One bucket j of a general step.  first: the step starts in bucket j (and not
 // before the trace); last: the step ends in bucket j (j < n_bins).
 template <class Acc>
