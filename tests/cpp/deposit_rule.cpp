@@ -64,9 +64,10 @@ struct CellAcc {
     int b0;
     const F* f1;
     const F* f2;
+    double scale;   // 2 for bucket units: the mixed window holds twice the sums
     void operator()(int j, double a, double b) const
     {
-        for (int c = 0; c < 6; c++) cells[6 * (j - b0) + c] += (F)a * f1[c] + (F)b * f2[c];
+        for (int c = 0; c < 6; c++) cells[6 * (j - b0) + c] += (F)(scale * a) * f1[c] + (F)(scale * b) * f2[c];
     }
 };
 
@@ -110,10 +111,10 @@ static void lane_walk(int NP, const int* idx, const double* ts, const float* u, 
             else
                 t = tg::step_common<F>(Rcp{}, common, ts[k], idx[k], ts[k + 1], i2, t0, dt, inv, nb, b0, b1, -1,
                                        -1);
-            const int ja = (unsigned)(idx[k] - b0) < (unsigned)nw ? idx[k] - b0 : -1;
-            tg::lane_step(L, add, common, gen || i2 != idx[k], t, ja, f1, f2);
+            const int ja = idx[k] - b0;
+            tg::lane_step(L, add, common, gen || i2 != idx[k], t, ja, (unsigned)ja < (unsigned)nw, f1, f2);
             if (gen) {
-                CellAcc<F> acc{cells.data(), b0, f1, f2};
+                CellAcc<F> acc{cells.data(), b0, f1, f2, units ? 2.0 : 1.0};
                 const int hi = b1 + ((spill && i2 == b1) ? 1 : 0);
                 if (units) {
                     const double q1 = (double)idx[k] + (double)u[k], q2 = (double)idx[k + 1] + (double)u[k + 1];
@@ -140,7 +141,7 @@ static void lane_walk(int NP, const int* idx, const double* ts, const float* u, 
             }
         }
         for (int j = 0; j < nw; j++)
-            for (int c = 0; c < 6; c++) trace[6 * (b0 + j) + c] += (double)cells[6 * j + c];
+            for (int c = 0; c < 6; c++) trace[6 * (b0 + j) + c] += (units ? 0.5 : 1.0) * (double)cells[6 * j + c];
         carry = spill;
     }
 }

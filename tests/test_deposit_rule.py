@@ -1,7 +1,9 @@
 """The product's bucket-deposit rule (teufel_amd/csrc/tg_deposit.h, the code the
 HIP kernels run) compiled for the host and compared with the oracle's
 restatement of src/particle.cpp:788-865 on 200k random steps and window
-splits (tests/cpp/deposit_rule.cpp).  CPU only."""
+splits, and the per-lane walk's bucket sums (the lane accumulator) on 20k
+random whole trajectories through windows with and without the spill bucket
+(tests/cpp/deposit_rule.cpp).  CPU only."""
 import os
 import subprocess
 

@@ -331,6 +331,11 @@ struct Window {
     }
 };
 
+// The mixed kernel's FP32 window holds twice the bucket sums (its common
+// weights come doubled from step_common_units), halved when it is flushed.
+template <class F>
+constexpr double kCellScale = sizeof(F) == 4 ? 2.0 : 1.0;
+
 // Accumulator for step_general: absolute bucket j -> window cell j - b0.
 template <class F>
 struct WindowAcc {
@@ -341,7 +346,7 @@ struct WindowAcc {
     __device__ __forceinline__ void operator()(int j, double a, double b) const
     {
         auto* c0 = w.cell(j - b0);
-        const F fa = (F)a, fb = (F)b;
+        const F fa = (F)(kCellScale<F> * a), fb = (F)(kCellScale<F> * b);
 #pragma unroll
         for (int c = 0; c < 6; ++c) Window<F>::add(c0 + c * kLanes, fa * f1[c] + fb * f2[c]);
     }
@@ -507,8 +512,9 @@ __device__ __forceinline__ void deposit_lane(const Window<double>& win, LaneAcc<
     const bool common = valid && !gen;
     const TwoBuckets<double> t = step_common<double>(RcpAmp{}, common, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt,
                                                      n_bins, b0, b1, -1, -1);
-    const int ja = (unsigned)(p1.idx - b0) < (unsigned)(b1 - b0) ? p1.idx - b0 : -1;
-    lane_step(L, WindowAdd<double>{win}, common, walking && (gen || i2 != p1.idx), t, ja, p1.f, p2.f);
+    const int ja = p1.idx - b0;
+    lane_step(L, WindowAdd<double>{win}, common, walking && (gen || i2 != p1.idx), t, ja,
+              (unsigned)ja < (unsigned)(b1 - b0), p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
             WindowAcc<double> acc{win, b0, p1.f, p2.f};
@@ -530,8 +536,9 @@ __device__ __forceinline__ void deposit_lane(const Window<float>& win, LaneAcc<f
     const bool common = valid && !gen;
     const TwoBuckets<float> t = step_common_units<float>(RcpF32{}, common, p1.idx, p1.u, p2.idx, p2.u, i2, b0, b1,
                                                          -1, -1);
-    const int ja = (unsigned)(p1.idx - b0) < (unsigned)(b1 - b0) ? p1.idx - b0 : -1;
-    lane_step(L, WindowAdd<float>{win}, common, walking && (gen || i2 != p1.idx), t, ja, p1.f, p2.f);
+    const int ja = p1.idx - b0;
+    lane_step(L, WindowAdd<float>{win}, common, walking && (gen || i2 != p1.idx), t, ja,
+              (unsigned)ja < (unsigned)(b1 - b0), p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
             WindowAcc<float> acc{win, b0, p1.f, p2.f};
@@ -860,24 +867,18 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
         eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
         deposit_lane(win, L, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, spill);
         ++steps;
-        if (walking) {
-            if (pe.idx >= b1) {
-                walking = false;
-                if (spill && pe.idx == b1) {   // deposited whole, the rest of it into the spill bucket
-                    ++kl;
-                    idx0 = pe.idx;
-                    lane_fold(L, pe.f);
-                    lane_close(L, wadd, b1 - b0);
-                }
-            } else {
-                ++kl;
-                idx0 = pe.idx;
-                walking = kl < NP - 1;
-                if (!walking) {   // the particle's last point
-                    lane_fold(L, pe.f);
-                    if ((unsigned)(idx0 - b0) < (unsigned)(b1 - b0)) lane_close(L, wadd, idx0 - b0);
-                }
-            }
+        // The lane moves past the step if it ends inside the window, or, with
+        // spill, in bucket b1 (deposited whole, the rest of it into the spill
+        // bucket); it walks on from inside the window.  A lane that stops
+        // past its last step adds its bucket sum to the window: the spill
+        // bucket, or the bucket of the particle's last point.
+        const bool past = walking && (pe.idx < b1 || (spill && pe.idx == b1));
+        kl += past ? 1 : 0;
+        idx0 = past ? pe.idx : idx0;
+        walking = past && pe.idx < b1 && kl < NP - 1;
+        if (past && !walking) {
+            lane_fold(L, pe.f);
+            if ((unsigned)(idx0 - b0) < (unsigned)(b1 - b0 + (spill ? 1 : 0))) lane_close(L, wadd, idx0 - b0);
         }
     };
     for (;;) {
@@ -1017,7 +1018,7 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         // for the magnetic components)
         auto cell = [&](int j, int c) {
             const double v = (double)win.lane_base[(6 * j + c) * kLanes];
-            return (PREC == 1 && c >= 3) ? v * kInvSpeedOfLight : v;
+            return PREC == 1 ? v * (c >= 3 ? 0.5 * kInvSpeedOfLight : 0.5) : v;
         };
         auto store = [&](int j, bool add) {
             add = add || it.chunk == 0;

@@ -81,7 +81,10 @@ TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, 
 // and no weight needs a select: a single-bucket step gets lb = 0 from
 // x2 - e = 0.  A zero-length step adds zeros (its length is raised to
 // 1e-30 before the reciprocal).  idx2 is the unclamped index of ts2
-// (i2 = min(idx2, n_bins)); [b0, b1) lies within the trace.
+// (i2 = min(idx2, n_bins)); [b0, b1) lies within the trace.  The weights are
+// twice those of step_common (the mixed kernel's window holds twice the
+// bucket sums and halves them when it is flushed: a power of two, so every
+// rounding is the same, and the weights need no factor 1/2).
 template <class F, class Rcp>
 TG_HD inline TwoBuckets<F> step_common_units(const Rcp& rcp, bool common, int idx1, F u1, int idx2, F u2, int i2,
                                              int b0, int b1, int none_a, int none_b)
@@ -94,11 +97,10 @@ TG_HD inline TwoBuckets<F> step_common_units(const Rcp& rcp, bool common, int id
     const F len = x2 - u1;
     const F idts = rcp(len > (F)1e-30 ? len : (F)1e-30);
     const F la = da * idts, lb = db * idts;
-    const F ha = (F)0.5 * da, hb = (F)0.5 * db;
-    o.aa = ha * ((F)1 + lb);
-    o.ba = ha * la;
-    o.ab = hb * lb;
-    o.bb = hb * (la + (F)1);
+    o.aa = da * ((F)1 + lb);
+    o.ba = da * la;
+    o.ab = db * lb;
+    o.bb = db * (la + (F)1);
     const unsigned span = (unsigned)(b1 - b0);
     o.ja = (common && (unsigned)(idx1 - b0) < span) ? idx1 - b0 : none_a;
     o.jb = (common && !single && (unsigned)(i2 - b0) < span) ? i2 - b0 : none_b;
@@ -138,17 +140,17 @@ TG_HD inline void lane_begin(LaneAcc<F>& L)
 // with weights t; any other step adds nothing here (a general one goes to
 // step_general), and its end point starts the next bucket sum from zero.
 // leaves: the step does not end in bucket idx1 (crossing or general); ja: the
-// window-relative index of bucket idx1, negative outside the window.
-// add(j, c, v) adds v to component c of window bucket j.
+// window-relative index of bucket idx1, in_win: whether it lies in the
+// window.  add(j, c, v) adds v to component c of window bucket j.
 template <class F, class Add>
 TG_HD inline void lane_step(LaneAcc<F>& L, const Add& add, bool common, bool leaves, const TwoBuckets<F>& t, int ja,
-                            const F* f1, const F* f2)
+                            bool in_win, const F* f1, const F* f2)
 {
     const F aa = common ? t.aa : (F)0, ba = common ? t.ba : (F)0;
     const F wa = L.wc + aa;
     for (int c = 0; c < 6; ++c) L.acc[c] = fma_f(wa, f1[c], L.acc[c]);
     if (leaves) {
-        if (ja >= 0)
+        if (in_win)
             for (int c = 0; c < 6; ++c) add(ja, c, fma_f(ba, f2[c], L.acc[c]));
         const F ab = common ? t.ab : (F)0;
         for (int c = 0; c < 6; ++c) L.acc[c] = ab * f1[c];
@@ -231,6 +233,7 @@ TG_HD inline void deposit_step(const Acc& acc, const Rcp& rcp, bool active, doub
 // q = idx + u (step_common_units); the trace spans q in (-1/2, n_bins - 1/2),
 // so the window test and case A become index tests, and a general step runs
 // step_general with t0 = 0, dt = 1.  rcp: reciprocal in F, rcpd in double.
+// acc receives the rule's own weights (step_common_units' halved).
 template <class F, class Acc, class Rcp, class RcpD>
 TG_HD inline void deposit_step_units(const Acc& acc, const Rcp& rcp, const RcpD& rcpd, bool active, int idx1, F u1,
                                      int idx2, F u2, int n_bins, int b0, int b1)
@@ -244,8 +247,8 @@ TG_HD inline void deposit_step_units(const Acc& acc, const Rcp& rcp, const RcpD&
         return;
     }
     const TwoBuckets<F> t = step_common_units<F>(rcp, true, idx1, u1, idx2, u2, i2, b0, b1, -1, -1);
-    if (t.ja >= 0) acc(t.ja + b0, t.aa, t.ba);
-    if (t.jb >= 0) acc(t.jb + b0, t.ab, t.bb);
+    if (t.ja >= 0) acc(t.ja + b0, (F)0.5 * t.aa, (F)0.5 * t.ba);
+    if (t.jb >= 0) acc(t.jb + b0, (F)0.5 * t.ab, (F)0.5 * t.bb);
 }
 
 }  // namespace tg
