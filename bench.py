@@ -300,6 +300,7 @@ def main():
     ap.add_argument("--groups", type=int, default=0, help="window groups per (tile, chunk); 0 = automatic")
     ap.add_argument("--chunk", type=int, default=0, help="particles per chunk; 0 = automatic")
     ap.add_argument("--walk", type=int, default=0, help="1 = wave-uniform walk, 2 = per-lane walk, 0 = default")
+    ap.add_argument("--spill", type=int, default=0, help="spill bucket: 1 = on, 2 = off, 0 = default (on)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -334,9 +335,10 @@ def main():
     else:
         timed, warm = [None] * args.steps, [None] * args.warmup
     tuning = None
-    if args.window or args.graph or args.groups or args.chunk or args.walk:
+    if args.window or args.graph or args.groups or args.chunk or args.walk or args.spill:
         tuning = {"bins_per_window": args.window, "use_graph": 1 if args.graph else 0,
-                  "window_groups": args.groups, "particles_per_chunk": args.chunk, "walk": args.walk}
+                  "window_groups": args.groups, "particles_per_chunk": args.chunk, "walk": args.walk,
+                  "spill": args.spill}
     tuned = None
     if not args.no_autotune and not args.window:
         t = time.time()
