@@ -34,11 +34,13 @@ def program():
         pytest.skip("no C++ compiler")
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
     srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    tmp = "%s.%d" % (BIN, os.getpid())
     cmd = [cxx, "-std=c++14", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
-           "-fno-sanitize-recover=undefined", "-I" + HOST, "-o", BIN] + srcs + [
+           "-fno-sanitize-recover=undefined", "-I" + HOST, "-o", tmp] + srcs + [
         os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp"), os.path.join(ROOT, "tests", "cpp", "tg_stub.cpp")]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
+    os.replace(tmp, BIN)   # whole: parallel test workers may build it at once
     return BIN
 
 
@@ -87,10 +89,12 @@ def test_thread_sanitizer_on_combined_calls(tmp_path):
         pytest.skip("no C++ compiler")
     exe = os.path.join(ROOT, "tests", "cpp", "_build", "host_api_test_tsan")
     srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
-    r = subprocess.run([cxx, "-std=c++14", "-O1", "-g", "-fsanitize=thread", "-I" + HOST, "-o", exe] + srcs + [
+    tmp = "%s.%d" % (exe, os.getpid())
+    r = subprocess.run([cxx, "-std=c++14", "-O1", "-g", "-fsanitize=thread", "-I" + HOST, "-o", tmp] + srcs + [
         os.path.join(ROOT, "tests", "cpp", "host_api_test.cpp"), os.path.join(ROOT, "tests", "cpp", "tg_stub.cpp")],
         capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
+    os.replace(tmp, exe)
     from test_host_api import _perpixel_input
     inp = str(tmp_path / "in.bin")
     _perpixel_input(inp, n_particles=2, n_steps=50, npx=8, n_bins=16)

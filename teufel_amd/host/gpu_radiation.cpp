@@ -242,7 +242,15 @@ void integrate(const std::vector<ChargedParticle*>& particles, const std::vector
         std::vector<Request*> batch;
         batch.swap(s.queue);
         lk.unlock();
-        launch_batch(s, batch);
+        try {
+            launch_batch(s, batch);
+        } catch (...) {
+            // (host allocation failures outside a launch) every request of
+            // the batch gets the error, and the next caller may launch
+            const std::exception_ptr err = std::current_exception();
+            for (Request* q : batch)
+                if (!q->error) q->error = err;
+        }
         lk.lock();
         for (Request* q : batch) q->done = true;
         s.launching = false;
