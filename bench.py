@@ -11,10 +11,12 @@ Default workload: c5, the largest configuration (2e5 macro-electrons of
 steps are K particle batches that together cover the whole bunch once:
 step i integrates particles [i B, (i+1) B), B = ceil(n / K), through
 tg_integrate_range into the one trace store, as a caller streaming batches
-would.  The W warm-up steps run the first batches into the store, which is
-zeroed before the timed steps, so after them the store holds the full
+would; the first warm-up step captures the launch in a hipGraph and every
+later step replays it, shifted to its batch (--no-graph: eager launches).
+The W warm-up steps run the first batches into the store, which is zeroed
+before the timed steps, so after them the store holds the full
 mixed-precision result.  `value` = pairs per step / median step time.  The
-FP64 kernel then runs the same K batches into a zeroed store (fp64_value),
+FP64 kernel then runs the same K batches, the same way, into a zeroed store (fp64_value),
 and the full mixed result's per-pixel energy error against the full FP64
 result is reported (eps_W_mixed_vs_fp64, bound 1e-5).
 
@@ -297,6 +299,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the launch sequence from a hipGraph")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="eager launches for the batched workloads (their default is hipGraph replays)")
     ap.add_argument("--groups", type=int, default=0, help="window groups per (tile, chunk); 0 = automatic")
     ap.add_argument("--chunk", type=int, default=0, help="particles per chunk; 0 = automatic")
     ap.add_argument("--walk", type=int, default=0, help="1 = wave-uniform walk, 2 = per-lane walk, 0 = default")
@@ -334,9 +338,12 @@ def main():
             % (-(-w.n_particles // b), b, len(timed), len(warm)))
     else:
         timed, warm = [None] * args.steps, [None] * args.warmup
+    # batched steps replay one captured launch per batch size (the library
+    # shifts its particle range), unless --no-graph
+    graph = args.graph or (batched and not args.no_graph)
     tuning = None
-    if args.window or args.graph or args.groups or args.chunk or args.walk or args.spill:
-        tuning = {"bins_per_window": args.window, "use_graph": 1 if args.graph else 0,
+    if args.window or graph or args.groups or args.chunk or args.walk or args.spill:
+        tuning = {"bins_per_window": args.window, "use_graph": 1 if graph else 0,
                   "window_groups": args.groups, "particles_per_chunk": args.chunk, "walk": args.walk,
                   "spill": args.spill}
     tuned = None
@@ -344,7 +351,7 @@ def main():
         t = time.time()
         tuned = ctx.autotune(prec)
         log("autotune %s: %s in %.1f s" % (args.precision, tuned, time.time() - t))
-        if args.graph:
+        if graph:
             tuning = {"bins_per_window": 0, "use_graph": 1}
 
     run_steps(ctx, prec, warm, pairs_per_particle, tuning, "warmup")
@@ -378,7 +385,13 @@ def main():
             t = time.time()
             log("autotune fp64: %s in %.1f s" % (ctx.autotune(TG_FP64), time.time() - t))
         if batched:
-            st64 = run_steps(ctx, TG_FP64, timed, pairs_per_particle, None, "fp64 step")
+            # the same timing path as the mixed steps (warm-up captures)
+            tun64 = {"bins_per_window": 0, "use_graph": 1} if graph else None
+            if graph:
+                run_steps(ctx, TG_FP64, timed[:1], pairs_per_particle, tun64, "fp64 warmup")
+                for i in ids:
+                    ctx.zero_screen(i)
+            st64 = run_steps(ctx, TG_FP64, timed, pairs_per_particle, tun64, "fp64 step")
         else:
             n64 = args.fp64_steps or min(3, len(timed))
             st64 = run_steps(ctx, TG_FP64, [None] * n64, pairs_per_particle, None, "fp64 step")
@@ -410,9 +423,11 @@ def main():
     cfg = {"workload": w.name, "description": w.description, "particles": w.n_particles,
            "points_per_particle": w.n_points, "pixels": sum(s.n_pix for s in w.screens),
            "bins": s0.n_bins, "screens": len(w.screens), "n_pairs": n_pairs, "n_inwin": n_inwin,
-           "precision": args.precision, "tuning": tuned, "graph": bool(args.graph),
-           "step": ("one particle batch of %d through tg_integrate_range; the %d timed steps cover %d of %d "
-                    "particles" % (timed[0][1] - timed[0][0], len(timed), covered, w.n_particles))
+           "precision": args.precision, "tuning": tuned, "graph": bool(graph),
+           "step": ("one particle batch of %d through tg_integrate_range%s; the %d timed steps cover %d of %d "
+                    "particles" % (timed[0][1] - timed[0][0],
+                                   " (a replay of the captured launch, shifted to the batch)" if graph else "",
+                                   len(timed), covered, w.n_particles))
            if batched else "one whole tg_integrate"}
     ctx.close()
     if not args.no_c2 and w.name != "c2":
@@ -434,7 +449,10 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": read_traffic(PREC_NAME[prec], w.name),
                      "kernel": KERNEL_NAME[prec], "flop_per_pair": FLOP_PER_PAIR, "kernel_ms": kern_mean * 1e3,
-                     "pairs_per_launch": pairs_launch},
+                     "pairs_per_launch": pairs_launch,
+                     "timed_by": ("HIP events around each graph replay (the radiation kernel and the partial-"
+                                  "trace reduction after it)" if graph else
+                                  "HIP events around the radiation kernel on its stream")},
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -56,7 +56,9 @@ typedef struct {
     int bins_per_window;     /* trace bins held in LDS per wave at a time (>= 1, capped by the LDS
                                 size; 0: the autotuned or default window) */
     int particles_per_chunk; /* particles per work item (partial traces summed in order) */
-    int use_graph;           /* 1: replay the launch sequence from a captured hipGraph */
+    int use_graph;           /* 1: replay the launch sequence from a captured hipGraph; one capture
+                                serves every particle range of the same size (tg_integrate_range
+                                over equal batches replays it, shifted to each batch) */
     int window_groups;       /* windows of a (tile, chunk) split into this many work items */
     int walk;                /* 1: the wave walks each particle's steps together (lanes share a
                                 step); 2: every lane (pixel) walks its own steps; 0: default (2) */

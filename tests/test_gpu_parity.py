@@ -383,6 +383,40 @@ def test_graph_replay_after_eager_launch_of_another_plan(ctx):
             assert trace_rel_error(out, eager) <= 1e-5
 
 
+@pytest.mark.parametrize("prec", [TG_FP64, TG_MIXED])
+def test_graph_replays_over_particle_batches(ctx, prec):
+    say("one captured launch replayed over equal particle batches (shifted ranges), prec=%d: bitwise equal to "
+        "eager batches; a smaller last batch captures its own launch" % prec)
+    tr, obs, t0, dt, _ = _small_bunch(n_particles=27, n_steps=1500)
+    ctx.remove_screens()
+    ctx.set_trajectories(tr)
+    sid = ctx.add_screen(obs, t0, dt, 256)
+    batches = [(0, 8), (8, 16), (16, 24), (24, 27), (8, 16)]
+    tun = {"particles_per_chunk": 3, "bins_per_window": 6}   # several chunks: partial traces in the graph
+    eager = []
+    for p0, p1 in batches:
+        ctx.zero_screen(sid)
+        ctx.integrate_range(prec, p0, p1, tuning=tun)
+        eager.append(ctx.read_traces(sid))
+    ctx.zero_screen(sid)
+    for (p0, p1), ref in zip(batches, eager):
+        ctx.zero_screen(sid)
+        ctx.integrate_range(prec, p0, p1, tuning=dict(tun, use_graph=1))
+        out = ctx.read_traces(sid)
+        np.abs(out).sum()  # host work between replays
+        assert np.array_equal(out, ref), (p0, p1)
+    # the batches accumulate into one store as a streaming caller's would
+    ctx.zero_screen(sid)
+    for p0, p1 in batches[:4]:
+        ctx.integrate_range(prec, p0, p1, tuning=dict(tun, use_graph=1))
+    whole = ctx.read_traces(sid)
+    ref = oracle.integrate_screen(tr, obs, t0, dt, 256)
+    if prec == TG_FP64:
+        assert trace_rel_error(whole, ref) <= FP64_TOL
+    else:
+        assert trace_rel_error(whole, ref) <= 1e-4
+
+
 def test_autotune_failed_trial_restores_traces(ctx, monkeypatch):
     say("tg_autotune whose 3rd trial fails (TG_AUTOTUNE_FAIL_TRIAL): error reported, traces unchanged")
     tr, obs, t0, dt, _ = _small_bunch(n_particles=16, n_steps=1000)

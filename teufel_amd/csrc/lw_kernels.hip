@@ -1093,7 +1093,10 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
 // the cursors of its current item in its own slot of slot_store (slot_bytes
 // per wave: slot_particles int2 cursors, then, when lane_offsets, kLanes
 // bytes per particle).  The grid is sized to the resident waves, so the
-// cursor store scales with the machine, not with the work list.
+// cursor store scales with the machine, not with the work list.  p_shift
+// (when not null) is read at launch and added to every item's particle
+// range, so that a captured launch replays over another particle batch of
+// the same size.
 template <int PREC, bool LANES>
 __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restrict__ items, int n_items,
                                                         int* __restrict__ next_item,
@@ -1101,8 +1104,10 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
                                                         const typename RecOf<PREC>::type* __restrict__ recs,
                                                         int n_points, unsigned char* __restrict__ slot_store,
                                                         size_t slot_bytes, int slot_particles, int lane_offsets,
-                                                        unsigned long long* __restrict__ stats, int lds_window)
+                                                        unsigned long long* __restrict__ stats, int lds_window,
+                                                        const int* __restrict__ p_shift)
 {
+    const int shift = p_shift ? __builtin_amdgcn_readfirstlane(*p_shift) : 0;
     unsigned char* slot = slot_store + (size_t)blockIdx.x * slot_bytes;
     int2* cur_base = reinterpret_cast<int2*>(slot);
     unsigned char* lane_cur = lane_offsets ? slot + (size_t)slot_particles * sizeof(int2) : nullptr;
@@ -1111,7 +1116,9 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
         if (threadIdx.x == 0) i = atomicAdd(next_item, 1);
         i = __builtin_amdgcn_readfirstlane(i);
         if (i >= n_items) break;
-        const WorkItem it = items[i];
+        WorkItem it = items[i];
+        it.p0 += shift;
+        it.p1 += shift;
         unsigned long long n_steps = 0, n_walks = 0;
         run_item<PREC, LANES>(it, screens, recs, n_points, cur_base, lane_cur, lds_window, n_steps, n_walks);
         if (stats && threadIdx.x == 0) {
@@ -1122,13 +1129,17 @@ __global__ __launch_bounds__(64) void radiation_kernel(const WorkItem* __restric
 }
 
 template __global__ void radiation_kernel<0, false>(const WorkItem*, int, int*, const ScreenDev*, const RecF64*, int,
-                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int,
+                                                    const int*);
 template __global__ void radiation_kernel<1, false>(const WorkItem*, int, int*, const ScreenDev*, const RecMixed*, int,
-                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int,
+                                                    const int*);
 template __global__ void radiation_kernel<0, true>(const WorkItem*, int, int*, const ScreenDev*, const RecF64*, int,
-                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int,
+                                                    const int*);
 template __global__ void radiation_kernel<1, true>(const WorkItem*, int, int*, const ScreenDev*, const RecMixed*, int,
-                                                    unsigned char*, size_t, int, int, unsigned long long*, int);
+                                                    unsigned char*, size_t, int, int, unsigned long long*, int,
+                                                    const int*);
 
 // ---------------------------------------------------------------------------
 // Add the partial traces of chunks 1.. into the trace store (which chunk 0

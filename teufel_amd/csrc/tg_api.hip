@@ -89,6 +89,8 @@ struct tg_context {
     size_t cap_slots = 0;
     int* d_next_item = nullptr;        // work-list counter of the persistent radiation kernel
     size_t cap_next_item = 0;
+    int* d_shift = nullptr;            // particle shift a captured launch reads (graph replays)
+    size_t cap_shift = 0;
     double* d_partial = nullptr;
     size_t cap_partial = 0;
     // walk statistics (diagnostic; on when TG_STATS=1 in the environment)
@@ -109,10 +111,12 @@ struct tg_context {
     // this device's static scratch array (tg_integrate_range_static)
     void* static_scratch = nullptr;
     std::string static_scratch_err;
-    // graph cache
+    // graph cache: the captured launch of key, over particles from graph_p0;
+    // a replay over another range of the same size shifts them (d_shift)
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::string graph_key;
+    int graph_p0 = 0;
 };
 
 namespace {
@@ -583,6 +587,7 @@ int upload_plan(tg_context* c, LaunchPlan& plan)
     if ((rc = ensure_cap(c, c->d_partial, c->cap_partial, plan.partial_doubles))) return rc;
     if ((rc = ensure_cap(c, c->d_slots, c->cap_slots, (size_t)plan.slots * plan.slot_bytes))) return rc;
     if ((rc = ensure_cap(c, c->d_next_item, c->cap_next_item, 1))) return rc;
+    if ((rc = ensure_cap(c, c->d_shift, c->cap_shift, 1))) return rc;
     if ((rc = ensure_cap(c, c->d_items, c->cap_items, plan.items.size()))) return rc;
     if ((rc = ensure_cap(c, c->d_sdev, c->cap_sdev, plan.sdev.size()))) return rc;
     if (c->stats_on && (rc = ensure_cap(c, c->d_stats, c->cap_stats, 2 * plan.items.size()))) return rc;
@@ -606,6 +611,7 @@ struct LaunchBuffers {
     size_t slot_bytes;
     int lane_offsets;
     unsigned long long* stats;
+    const int* p_shift;   // added to the items' particle ranges at launch (null: none)
 };
 
 // Queue the radiation kernel of plan on the context's stream, the work-list
@@ -623,7 +629,8 @@ int launch_radiation(tg_context* c, int prec, const LaunchPlan& plan, const Laun
 #define TG_LAUNCH(P, L, R)                                                                                        \
     radiation_kernel<P, L><<<grid, kLanes, plan.lds, c->stream>>>(b.items, n_items, b.next_item, b.sdev, R,        \
                                                                   c->n_points, b.slots, b.slot_bytes, plan.chunk, \
-                                                                  b.lane_offsets, b.stats, plan.max_window)
+                                                                  b.lane_offsets, b.stats, plan.max_window,     \
+                                                                  b.p_shift)
     if (prec == TG_FP64 && plan.lanes)
         TG_LAUNCH(0, true, r64);
     else if (prec == TG_FP64)
@@ -639,11 +646,11 @@ int launch_radiation(tg_context* c, int prec, const LaunchPlan& plan, const Laun
 }
 
 // Queue radiation kernel + chunk reductions (capturable: kernels only).
-int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events)
+int enqueue_plan(tg_context* c, int prec, const LaunchPlan& plan, bool events, const int* p_shift = nullptr)
 {
     if (plan.items.empty()) return TG_OK;
     const LaunchBuffers b{c->d_items, c->d_next_item, c->d_sdev, c->d_slots, plan.slot_bytes,
-                          plan.lane_cursors ? 1 : 0, c->stats_on ? c->d_stats : nullptr};
+                          plan.lane_cursors ? 1 : 0, c->stats_on ? c->d_stats : nullptr, p_shift};
     int rc;
     if ((rc = launch_radiation(c, prec, plan, b, events))) return rc;
     for (size_t i = 0; i < plan.sdev.size() && plan.n_chunks > 1; ++i) {
@@ -705,10 +712,15 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
     tg_tuning eff = effective_tuning(c, prec, tun);
     bool use_graph = eff.use_graph != 0;
     char keybuf[256];
-    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window,
-             eff.particles_per_chunk, eff.window_groups, eff.walk, eff.spill, screen_id, p0, p1, c->screens.size());
+    // the key holds the size of the particle range, not its start: a captured
+    // launch replays over any range of that size (the kernel adds d_shift to
+    // its items' particle ranges), as a caller streaming equal batches needs
+    snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%d/%d/%d/%d/%d/%zu", prec, eff.bins_per_window,
+             eff.particles_per_chunk, eff.window_groups, eff.walk, eff.spill, screen_id, p1 - p0, c->screens.size());
     std::string key(keybuf);
     if (use_graph && c->gexec && c->graph_key == key) {
+        set_int_kernel<<<1, 64, 0, c->stream>>>(c->d_shift, p0 - c->graph_p0);
+        TG_HIP(c, hipGetLastError());
         TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         TG_HIP(c, hipGraphLaunch(c->gexec, c->stream));
         TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
@@ -723,9 +735,11 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
     if ((rc = upload_plan(c, plan))) return rc;
     if (use_graph) {
         free_graph(c);
+        set_int_kernel<<<1, 64, 0, c->stream>>>(c->d_shift, 0);
+        TG_HIP(c, hipGetLastError());
         TG_HIP(c, hipStreamSynchronize(c->stream));
         TG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_plan(c, prec, plan, false);
+        rc = enqueue_plan(c, prec, plan, false, c->d_shift);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) {
@@ -736,6 +750,7 @@ int integrate_impl(tg_context* c, int prec, const tg_tuning* tun, int screen_id,
         c->graph = g;
         TG_HIP(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
         c->graph_key = key;
+        c->graph_p0 = p0;
         TG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         TG_HIP(c, hipGraphLaunch(c->gexec, c->stream));
         TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
@@ -858,7 +873,7 @@ int integrate_static(tg_context* c, int prec, const tg_tuning* tun, int screen_i
                              hipMemcpyHostToDevice, c->stream));
     const LaunchBuffers b{reinterpret_cast<const WorkItem*>(base + off_items), reinterpret_cast<int*>(base),
                           reinterpret_cast<const ScreenDev*>(base + off_sdev), base + off_slots, slot_size(offsets),
-                          offsets ? 1 : 0, nullptr};
+                          offsets ? 1 : 0, nullptr, nullptr};
     if ((rc = launch_radiation(c, prec, plan, b, true))) return rc;
     TG_HIP(c, hipEventRecord(c->ev[2], c->stream));
     TG_HIP(c, hipEventSynchronize(c->ev[2]));   // the host copies' sources live until here
@@ -956,6 +971,7 @@ int tg_destroy(tg_context* c)
     (void)hipFree(c->d_sdev);
     (void)hipFree(c->d_slots);
     (void)hipFree(c->d_next_item);
+    (void)hipFree(c->d_shift);
     (void)hipFree(c->d_partial);
     (void)hipFree(c->d_stats);
     (void)hipFree(c->d_flag);
