@@ -44,6 +44,11 @@
 //   host_api_test screenread <screen.h5> <out.dump>
 //       reads a ScreenObserver HDF5 file and writes its cells and traces as a
 //       plain dump (positions, cell t0, samples) for the Python tests
+//   host_api_test snaph5 <out.h5>
+//       a SnapshotObserver<Bunch> with known field values written by
+//       generateOutput() (the HDF5 file of src/snapshot.cpp:226-392) and read
+//       back with the HDF5 reader: dataset shapes, attributes, cell positions
+//       and every value must survive (no GPU needed)
 //
 // Exit status: number of failed checks (0 = pass), 100+ on usage/IO errors.
 #include <algorithm>
@@ -539,6 +544,39 @@ int run_screenh5(const char* out)
     return errors;
 }
 
+// SnapshotObserver HDF5 file written by generateOutput and read back
+int run_snaph5(const char* out)
+{
+    const Vector origin(-0.02, 0.01, 7.5), dx(0.003, 0.0, 0.001), dy(0.0, 0.004, 0.0);
+    const unsigned nx = 6, ny = 5;
+    const double t = 2.7e-8;
+    SnapshotObserver<Bunch> snap(out, origin, dx, dy, nx, ny, t);
+    auto value = [](unsigned ix, unsigned iy, int c) { return ix * 100.0 + iy + c * 1e-3 - 0.5; };
+    for (unsigned ix = 0; ix < nx; ix++)
+        for (unsigned iy = 0; iy < ny; iy++)
+            snap.setField(ix, iy, ElMagField(Vector(value(ix, iy, 0), value(ix, iy, 1), value(ix, iy, 2)),
+                                             Vector(value(ix, iy, 3), value(ix, iy, 4), value(ix, iy, 5))));
+    snap.generateOutput();
+    const H5LiteFile f(out);
+    int errors = 0;
+    const std::vector<uint64_t> pd = f.dims("ObservationPosition"), fd = f.dims("ElMagField");
+    if (pd != std::vector<uint64_t>{nx, ny, 3} || fd != std::vector<uint64_t>{nx, ny, 6}) errors++;
+    if (f.attr_int("ObservationPosition", "Nx") != nx || f.attr_int("ObservationPosition", "Ny") != ny) errors++;
+    if (f.attr_double("ElMagField", "t0") != t) errors++;
+    if (f.exists("ParticleCoordinates")) errors++;
+    const std::vector<double> pos = f.read_double("ObservationPosition"), fld = f.read_double("ElMagField");
+    for (unsigned ix = 0; ix < nx && errors == 0; ix++)
+        for (unsigned iy = 0; iy < ny; iy++) {
+            const Vector p = snap.CellPosition((int)ix, (int)iy);
+            const double* q = &pos[(ix * ny + iy) * 3];
+            if (q[0] != p.x || q[1] != p.y || q[2] != p.z) errors++;
+            for (int c = 0; c < 6; c++)
+                if (fld[(ix * ny + iy) * 6 + c] != value(ix, iy, c)) errors++;
+        }
+    std::printf("snaph5: %u x %u cells written and read back, errors %d\n", nx, ny, errors);
+    return errors;
+}
+
 // a ScreenObserver file as a plain dump: positions, per-cell t0
 // (src/screen.cpp:106-110), samples
 int run_screenread(const char* in, const char* out)
@@ -585,6 +623,8 @@ int main(int argc, char** argv)
             return run_screenh5(argv[2]);
         } else if (mode == "screenread" && argc >= 4) {
             return run_screenread(argv[2], argv[3]);
+        } else if (mode == "snaph5" && argc >= 3) {
+            return run_snaph5(argv[2]);
         } else if (mode == "diffraction" && argc >= 5) {
             if (set_precision(argc > 5 ? argv[5] : nullptr)) return 100;
             errors = run_diffraction(argv[2], argv[3], argv[4]);

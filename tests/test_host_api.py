@@ -169,6 +169,32 @@ def test_screen_observer_file(tmp_path):
     assert np.array_equal(f, 8000.0 + 60.0 + 15e-3 + 1e-6 * np.arange(6))
 
 
+def test_snapshot_observer_file(tmp_path):
+    """SnapshotObserver::generateOutput writes the HDF5 file of
+    src/snapshot.cpp:226-392 (ObservationPosition [Nx][Ny][3] with Nx, Ny;
+    ElMagField [Nx][Ny][6] with t0); it reads back identically, and h5dump
+    reads the same shapes, attributes and values."""
+    _program()
+    out = str(tmp_path / "snap.h5")
+    r = subprocess.run([BIN, "snaph5", out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    if not _h5dump():
+        pytest.skip("h5dump not available for the independent check")
+    hdr = subprocess.run([_h5dump(), "-H", out], capture_output=True, text=True, timeout=60, cwd="/tmp")
+    assert hdr.returncode == 0, hdr.stderr
+    for name, shape in (("ElMagField", "( 6, 5, 6 )"), ("ObservationPosition", "( 6, 5, 3 )")):
+        block = hdr.stdout.split('DATASET "%s"' % name)[1].split("DATASET")[0]
+        assert shape in block and "H5T_IEEE_F64LE" in block
+    assert _h5dump_values(out, "/ObservationPosition/Nx", "-a").tolist() == [6]
+    assert _h5dump_values(out, "/ObservationPosition/Ny", "-a").tolist() == [5]
+    assert _h5dump_values(out, "/ElMagField/t0", "-a").tolist() == [2.7e-8]
+    # cell (ix=5, iy=0): origin + dX (5 - 5/2) + dY (0 - 2) (src/snapshot.cpp:405-408)
+    p = _h5dump_values(out, "/ObservationPosition", "-d", ["-s", "5,0,0", "-c", "1,1,3"])
+    assert np.allclose(p, [-0.02 + 0.003 * 2.5, 0.01 - 0.004 * 2.0, 7.5 + 0.001 * 2.5], rtol=0, atol=1e-15)
+    f = _h5dump_values(out, "/ElMagField", "-d", ["-s", "5,4,0", "-c", "1,1,6"])
+    assert np.array_equal(f, 500.0 + 4.0 + 1e-3 * np.arange(6) - 0.5)
+
+
 def test_screen_file_rejects_malformed(tmp_path):
     """The ScreenObserver file reader raises IOexception (exit status 110 of
     the test program) for a MeshedScreen file, which lacks the Screen dataset."""

@@ -3,6 +3,7 @@
 
 #include "fieldtrace.h"
 #include "gpu_radiation.h"
+#include "h5lite.h"
 #include "screens.h"
 
 template <typename SourceT>
@@ -70,20 +71,31 @@ void SnapshotObserver<SourceT>::fromBuffer(double* buffer, std::size_t size)
 }
 
 template <typename SourceT>
-void SnapshotObserver<SourceT>::generateOutput()
+void SnapshotObserver<SourceT>::WriteFieldHDF5()
 {
-    std::vector<Vector> pts;
-    std::vector<FieldTrace*> traces;
-    std::vector<FieldTrace> store;
-    store.reserve(FieldArray.size());
+    H5LiteWriter w;
+    std::vector<double> pos;
+    pos.reserve((std::size_t)Nx * Ny * 3);
     for (unsigned int ix = 0; ix < Nx; ix++)
         for (unsigned int iy = 0; iy < Ny; iy++) {
-            pts.push_back(CellPosition((int)ix, (int)iy));
-            store.emplace_back(t_obs, 0.0, 1);
-            store.back().set_field(0, FieldArray[(std::size_t)ix * Ny + iy]);
+            const Vector p = CellPosition((int)ix, (int)iy);
+            pos.insert(pos.end(), {p.x, p.y, p.z});
         }
-    for (FieldTrace& t : store) traces.push_back(&t);
-    writeDump(FileName, pts, traces);
+    w.add_double("ObservationPosition", {Nx, Ny, 3}, pos);
+    w.attr_int32("ObservationPosition", "Nx", (int32_t)Nx);
+    w.attr_int32("ObservationPosition", "Ny", (int32_t)Ny);
+    double* buffer = getBuffer();
+    const std::vector<double> fld(buffer, buffer + getBufferSize());
+    delete[] buffer;
+    w.add_double("ElMagField", {Nx, Ny, 6}, fld);
+    w.attr_double("ElMagField", "t0", t_obs);
+    w.write(FileName);
+}
+
+template <typename SourceT>
+void SnapshotObserver<SourceT>::generateOutput()
+{
+    WriteFieldHDF5();
 }
 
 template class SnapshotObserver<Beam>;

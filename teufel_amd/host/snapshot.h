@@ -29,8 +29,13 @@ public:
     std::size_t getBufferSize() override { return (std::size_t)Nx * Ny * 6; }
     double* getBuffer() override;
     void fromBuffer(double* buffer, std::size_t size) override;
-    // plain binary dump of the cell positions and fields (writeDump format,
-    // one sample per point, t0 = the observation time)
+    // the HDF5 file of src/snapshot.cpp:226-392: ObservationPosition
+    // [Nx][Ny][3] (int attributes Nx, Ny) and ElMagField [Nx][Ny][6] (double
+    // attribute t0, the observation time).  The reference's optional
+    // ParticleCoordinates dataset holds particle positions the tracking
+    // stored at that time; there is no tracking here, so it is not written.
+    void WriteFieldHDF5();
+    // WriteFieldHDF5 (src/snapshot.cpp:398-402)
     void generateOutput() override;
 
 private:
