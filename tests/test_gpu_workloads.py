@@ -15,7 +15,7 @@ The device pusher generates the trajectories (it is checked against the
 oracle pusher elsewhere); both the HIP kernels and the oracle integrate those
 same trajectories.  Bounds: FP64 eps_E <= 1e-12 against the oracle, mixed
 eps_W <= 1e-5 per pixel against the oracle (energy through the pixel, pixels
->= 1e-3 of the maximum), the bounds of the north-star budget.  The long-sum
+>= 1e-3 of the maximum), the stated error budget.  The long-sum
 cases put 1000-2000 particles into every cell so that the mixed kernel's FP32
 window sums and their flushes run at the lengths of the full workloads.
 """
