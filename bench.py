@@ -130,15 +130,26 @@ def batch_ranges(n, k):
 
 
 def cpu_model():
-    """The host CPU's model name (/proc/cpuinfo)."""
+    """The host CPU's model name with its CPUID family, model and stepping
+    (/proc/cpuinfo of the first CPU).  A virtual machine may report a generic
+    name such as "AMD EPYC"; the three numbers still identify the core
+    generation (family 26 is Zen 5, 25 is Zen 3/4)."""
+    keys = {"model name": None, "cpu family": None, "model": None, "stepping": None}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break   # end of the first CPU's block
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in keys and keys[k] is None:
+                    keys[k] = v.strip()
     except OSError:
         pass
-    return "unknown"
+    name = keys["model name"] or "unknown"
+    if keys["cpu family"] is not None:
+        name += " (family %s, model %s, stepping %s)" % (keys["cpu family"], keys["model"], keys["stepping"])
+    return name
 
 
 def physical_cores():
