@@ -352,6 +352,30 @@ struct WindowAcc {
     }
 };
 
+// Accumulator for step_general in the per-lane walk: bucket j -> window cell
+// j - b0, and, when the window has a spill bucket (nw = b1 - b0 >= 0), bucket
+// b1 -> the lane's spill registers sp (run_item).  nw < 0: no spill bucket.
+template <class F>
+struct WindowAccSpill {
+    Window<F> w;
+    int b0, nw;
+    const F* f1;
+    const F* f2;
+    F* sp;
+    __device__ __forceinline__ void operator()(int j, double a, double b) const
+    {
+        const F fa = (F)(kCellScale<F> * a), fb = (F)(kCellScale<F> * b);
+        if (j - b0 == nw) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sp[c] = sp[c] + (fa * f1[c] + fb * f2[c]);
+        } else {
+            auto* c0 = w.cell(j - b0);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Window<F>::add(c0 + c * kLanes, fa * f1[c] + fb * f2[c]);
+        }
+    }
+};
+
 // s_waitcnt vmcnt(0) with expcnt and lgkmcnt left open (gfx9 encoding:
 // vmcnt[3:0] in bits 3:0 and [5:4] in bits 15:14, expcnt in 6:4, lgkmcnt in 11:8).
 constexpr int kWaitVmcnt0 = 0x0F70;
@@ -436,54 +460,48 @@ __device__ __forceinline__ RecF64 RecBuffer<RecF64>::load(int k) const
 // readlane: no LDS round trips).
 __device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
 
-// Deposit of step (p1 -> p2) into the window [b0, b1), FP64: retarded times
-// and bucket edges as the reference forms them.  With spill, the window has
-// one more LDS bucket, b1: a step that ends in bucket b1 (the walk's last
-// step for the lane) is deposited there too, whole, and is not walked again
-// in the next window, which takes that bucket over as its first; a step that
-// ends beyond b1 deposits only its buckets below b1 here (the next window
-// walks it again for the rest).  Walked steps start before b1, so a common
-// (one- or two-bucket) step reaches at most b1.
+// Deposit of step (p1 -> p2) into the window [b0, b1) (the wave-uniform
+// walk), FP64: retarded times and bucket edges as the reference forms them.
+// A step that ends at or beyond b1 deposits only its buckets below b1 here;
+// the next window walks it again for the rest.
 __device__ __forceinline__ void deposit_window(const Window<double>& win, bool active, const Point& p1,
                                                const Point& p2, double t0, double dt, double inv_dt, double tmin,
-                                               double tmax, int n_bins, int b0, int b1, bool spill)
+                                               double tmax, int n_bins, int b0, int b1)
 {
     const bool valid = active && p2.ts > tmin && p1.ts < tmax;    // particle.cpp:793
     const bool case_a = p1.ts <= tmin;
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
     const TwoBuckets<double> t = step_common<double>(RcpAmp{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
-                                                     inv_dt, n_bins, b0, b1 + (spill ? 1 : 0), -1, -1);
+                                                     inv_dt, n_bins, b0, b1, -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
             WindowAcc<double> acc{win, b0, p1.f, p2.f};
             step_general(acc, rcp_rn(p2.ts - p1.ts), case_a, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt, n_bins,
-                         b0, b1 + ((spill && i2 == b1) ? 1 : 0));
+                         b0, b1);
         }
     }
 }
 
 // Mixed: the same rule in bucket units (deposit_step_units), the common
-// step branch-free with FP32 weights, the general step under a ballot; spill
-// as for FP64.
+// step branch-free with FP32 weights, the general step under a ballot.
 __device__ __forceinline__ void deposit_window(const Window<float>& win, bool active, const PointM& p1,
                                                const PointM& p2, double, double, double, double, double,
-                                               int n_bins, int b0, int b1, bool spill)
+                                               int n_bins, int b0, int b1)
 {
     const bool valid = active && p2.idx >= 0 && p1.idx < n_bins;  // particle.cpp:793
     const bool case_a = p1.idx < 0;                               // particle.cpp:796
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
     const TwoBuckets<float> t = step_common_units<float>(RcpF32{}, valid && !gen, p1.idx, p1.u, p2.idx, p2.u, i2,
-                                                         b0, b1 + (spill ? 1 : 0), -1, -1);
+                                                         b0, b1, -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
             WindowAcc<float> acc{win, b0, p1.f, p2.f};
             const double q1 = (double)p1.idx + (double)p1.u, q2 = (double)p2.idx + (double)p2.u;
-            step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0,
-                         b1 + ((spill && i2 == b1) ? 1 : 0));
+            step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0, b1);
         }
     }
 }
@@ -496,14 +514,22 @@ struct WindowAdd {
 };
 
 // The per-lane walk's deposit of step (p1 -> p2): the rule of
-// deposit_window (same weights, spill and general steps), with the common
-// steps summed per bucket in the lane accumulator L (lane_step) instead of
-// added to the window step by step.  The accumulator of a lane that does not
-// walk the step takes garbage (it is closed when the lane stops, and the
-// next walk starts it afresh), so its update needs no mask.  FP64.
+// deposit_window (same weights and general steps), with the common steps
+// summed per bucket in the lane accumulator L (lane_step) instead of added to
+// the window step by step.  The accumulator of a lane that does not walk the
+// step takes garbage (it is closed when the lane stops, and the next walk
+// starts it afresh), so its update needs no mask.  With spill, the window has
+// a spill bucket b1 held in the lane's registers sp (run_item): a step that
+// ends in bucket b1 (the lane's last step in the window) is deposited whole,
+// its b1 part into sp, and is not walked again in the next window, which
+// takes the spill bucket over as its first; a step that ends beyond b1
+// deposits only its buckets below b1 here (the next window walks it again for
+// the rest).  Walked steps start before b1, so a common step reaches at most
+// b1, whose share stays in L until the lane stops.  FP64.
 __device__ __forceinline__ void deposit_lane(const Window<double>& win, LaneAcc<double>& L, bool walking,
                                              const Point& p1, const Point& p2, double t0, double dt, double inv_dt,
-                                             double tmin, double tmax, int n_bins, int b0, int b1, bool spill)
+                                             double tmin, double tmax, int n_bins, int b0, int b1, bool spill,
+                                             double* sp)
 {
     const bool valid = walking && p2.ts > tmin && p1.ts < tmax;    // particle.cpp:793
     const bool case_a = p1.ts <= tmin;
@@ -517,7 +543,7 @@ __device__ __forceinline__ void deposit_lane(const Window<double>& win, LaneAcc<
               (unsigned)ja < (unsigned)(b1 - b0), p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
-            WindowAcc<double> acc{win, b0, p1.f, p2.f};
+            WindowAccSpill<double> acc{win, b0, spill ? b1 - b0 : -1, p1.f, p2.f, sp};
             step_general(acc, rcp_rn(p2.ts - p1.ts), case_a, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt, n_bins,
                          b0, b1 + ((spill && i2 == b1) ? 1 : 0));
         }
@@ -527,7 +553,7 @@ __device__ __forceinline__ void deposit_lane(const Window<double>& win, LaneAcc<
 // Mixed: the same in bucket units (deposit_window for the mixed kernel).
 __device__ __forceinline__ void deposit_lane(const Window<float>& win, LaneAcc<float>& L, bool walking,
                                              const PointM& p1, const PointM& p2, double, double, double, double,
-                                             double, int n_bins, int b0, int b1, bool spill)
+                                             double, int n_bins, int b0, int b1, bool spill, float* sp)
 {
     const bool valid = walking && p2.idx >= 0 && p1.idx < n_bins;  // particle.cpp:793
     const bool case_a = p1.idx < 0;                               // particle.cpp:796
@@ -541,7 +567,7 @@ __device__ __forceinline__ void deposit_lane(const Window<float>& win, LaneAcc<f
               (unsigned)ja < (unsigned)(b1 - b0), p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
-            WindowAcc<float> acc{win, b0, p1.f, p2.f};
+            WindowAccSpill<float> acc{win, b0, spill ? b1 - b0 : -1, p1.f, p2.f, sp};
             const double q1 = (double)p1.idx + (double)p1.u, q2 = (double)p2.idx + (double)p2.u;
             step_general(acc, rcp_rn(q2 - q1), case_a, q1, p1.idx, q2, i2, 0.0, 1.0, 1.0, n_bins, b0,
                          b1 + ((spill && i2 == b1) ? 1 : 0));
@@ -719,7 +745,7 @@ __device__ __forceinline__ int walk_particle(const char* __restrict__ rp, int k,
             break;
         }
         eval_point<PREC>(src.next(k + 1), Px, Py, Pz, t0, dt, inv_dt, p1);
-        deposit_window(win, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, false);
+        deposit_window(win, active, p0, p1, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1);
         ++steps;
         if (next_k < 0 && __ballot(active && p1.idx >= b1)) {
             next_k = k;
@@ -780,17 +806,17 @@ __device__ __forceinline__ int point_index(const RecBuffer<typename RecOf<PREC>:
 // from nk.  A searched start, and a stored offset below the saturation, is
 // exactly where the skip would stop (the point after it lies at or beyond
 // b0), so only lanes starting at the wave's cursor or at a saturated offset
-// skip.  With spill (deposit_window), a lane whose last step ends in bucket
-// b1 has deposited it whole and stops at the step after it; carry says that
-// the previous window of the item spilled, so that the skip passes such a
-// step.
+// skip.  With spill (deposit_lane), a lane whose last step ends in bucket
+// b1 has deposited it whole, its b1 part into the spill registers sp, and
+// stops at the step after it; carry says that the previous window of the
+// item spilled, so that the skip passes such a step.
 template <int PREC>
 __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, int NP,
                                           bool active, double Px, double Py, double Pz, double t0, double dt,
                                           double inv_dt, double tmin, double tmax, int n_bins, int b0, int b1,
                                           const Window<typename RecOf<PREC>::amp>& win, bool search,
                                           const unsigned char* __restrict__ lc_in, unsigned char* __restrict__ lc_out,
-                                          bool spill, bool carry, int& nk, int& nmin)
+                                          bool spill, bool carry, typename RecOf<PREC>::amp* sp, int& nk, int& nmin)
 {
     using Rec = typename RecOf<PREC>::type;
     const RecBuffer<Rec> rbuf(rp, NP);
@@ -854,8 +880,8 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
     Pt pb;
     Rec rb;
     // The lane's bucket sum (deposit_lane), added to the window as the lane
-    // stops after a step that took it into bucket idx0: the spill bucket, or
-    // the bucket of the particle's last point.
+    // stops after a step that took it into bucket idx0: the spill bucket
+    // (the registers sp), or the bucket of the particle's last point.
     using F = typename RecOf<PREC>::amp;
     LaneAcc<F> L;
     lane_begin(L);
@@ -865,7 +891,7 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
         __builtin_amdgcn_sched_barrier(0);
         rnext = rbuf.load(kl + 2);
         eval_point<PREC>(rc, Px, Py, Pz, t0, dt, inv_dt, pe);
-        deposit_lane(win, L, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, spill);
+        deposit_lane(win, L, walking, ps, pe, t0, dt, inv_dt, tmin, tmax, n_bins, b0, b1, spill, sp);
         ++steps;
         // The lane moves past the step if it ends inside the window, or, with
         // spill, in bucket b1 (deposited whole, the rest of it into the spill
@@ -878,7 +904,13 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
         walking = past && pe.idx < b1 && kl < NP - 1;
         if (past && !walking) {
             lane_fold(L, pe.f);
-            if ((unsigned)(idx0 - b0) < (unsigned)(b1 - b0 + (spill ? 1 : 0))) lane_close(L, wadd, idx0 - b0);
+            const int j = idx0 - b0;
+            if (spill && j == b1 - b0) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) sp[c] = sp[c] + L.acc[c];
+            } else if ((unsigned)j < (unsigned)(b1 - b0)) {
+                lane_close(L, wadd, j);
+            }
         }
     };
     for (;;) {
@@ -982,48 +1014,59 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         }
     }
     // Spill bucket (per-lane walk): a window that is not the item's last has
-    // one more LDS bucket, its b1, which receives the walks' last steps that
-    // end there (deposit_window); the next window starts with it as its bucket
-    // 0.  carry: bucket 0 of this window holds the previous window's spill
-    // bucket; carry_stored: part of that bucket is already in the partial
-    // trace (the mixed kernel flushes the spill bucket with the window);
-    // carry_terms: the terms it may hold since.
+    // one more bucket, its b1, which receives the walks' last steps that end
+    // there (deposit_lane); the next window starts with it as its bucket 0.
+    // Each lane adds only to its own pixel's spill bucket, so it lives in the
+    // lane's registers sp, not in LDS: the window's LDS, which sets how many
+    // waves fit a CU, holds its W buckets only.  carry: bucket 0 of this
+    // window holds the previous window's spill bucket; carry_stored: part of
+    // that bucket is already in the partial trace (the mixed kernel flushes
+    // the spill bucket with the window); carry_terms: the terms it may hold
+    // since.
+    F sp[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sp[c] = (F)0;
     bool carry = false, carry_stored = false;
     unsigned carry_terms = 0;
     for (int b0 = it.bin_lo; b0 < it.bin_hi; b0 += W) {
         const int b1 = b0 + W < it.bin_hi ? b0 + W : it.bin_hi;
         const int nw = b1 - b0;
         const bool spill = LANES && S.spill && b1 < it.bin_hi;
-        const int ncell = (nw + (spill ? 1 : 0)) * 6;
+        const int ncell = nw * 6;
         // chunk 0 adds into the trace store, the others store partial traces
         const size_t px = (size_t)(active ? pix : 0);
         double* out = it.chunk == 0 ? S.traces + (px * nb + b0) * 6
                                     : S.partial + (((size_t)(it.chunk - 1) * S.n_pix + px) * nb + b0) * 6;
         if (carry) {
-            // the previous window was a full one (only those spill): its
-            // spill bucket is bucket W
+            // the previous window's spill bucket becomes bucket 0
 #pragma unroll
-            for (int c = 0; c < 6; ++c) win.lane_base[c * kLanes] = win.lane_base[(6 * W + c) * kLanes];
+            for (int c = 0; c < 6; ++c) win.lane_base[c * kLanes] = sp[c];
             for (int j = 6; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
         } else {
             for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
         }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sp[c] = (F)0;
         bool first_flush = true;
         bool b0_stored = carry && carry_stored;   // out of bucket 0 already holds part of it
         bool spill_stored = false;                // out of the spill bucket (bucket nw) holds part of it
         bool dirty = carry;    // the window holds terms not yet in the partial trace
         // estimated terms per cell since the last flush, times nw
         unsigned terms = carry ? carry_terms : 0;
-        // cell j of the window as an FP64 trace value (mixed cells hold c B
-        // for the magnetic components)
-        auto cell = [&](int j, int c) {
-            const double v = (double)win.lane_base[(6 * j + c) * kLanes];
+        // component c of a window cell as an FP64 trace value (mixed cells
+        // hold twice the sums, and c B for the magnetic components)
+        auto value = [&](F raw, int c) {
+            const double v = (double)raw;
             return PREC == 1 ? v * (c >= 3 ? 0.5 * kInvSpeedOfLight : 0.5) : v;
         };
+        // bucket j < nw (LDS), or j = nw (the spill registers), into out
         auto store = [&](int j, bool add) {
             add = add || it.chunk == 0;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[6 * j + c] = add ? out[6 * j + c] + cell(j, c) : cell(j, c);
+            for (int c = 0; c < 6; ++c) {
+                const double v = value(j == nw ? sp[c] : (F)win.lane_base[(6 * j + c) * kLanes], c);
+                out[6 * j + c] = add ? out[6 * j + c] + v : v;
+            }
         };
         auto flush = [&]() {
             if (active) {
@@ -1031,6 +1074,8 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                 if (spill) store(nw, spill_stored);
             }
             for (int j = 0; j < ncell; ++j) win.lane_base[j * kLanes] = (F)0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sp[c] = (F)0;
             first_flush = false;
             spill_stored = spill_stored || spill;
             dirty = false;
@@ -1055,7 +1100,7 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
                     unsigned char* lc = lane_cur ? lane_cur + (size_t)(p - it.p0) * kLanes : nullptr;
                     ws = walk_lanes<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt,
                                           inv_dt, tmin, tmax, nb, b0, b1, win, first, first ? nullptr : lc,
-                                          b1 < it.bin_hi ? lc : nullptr, spill, carry, nk, nmin);
+                                          b1 < it.bin_hi ? lc : nullptr, spill, carry, sp, nk, nmin);
                 } else {
                     const bool verify = first && k > 0;
                     ws = walk_particle<PREC>(particle_records<PREC>(recs, p, NP), k, NP, active, Px, Py, Pz, t0, dt, inv_dt, tmin,
@@ -1074,7 +1119,7 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
             if (b1 < it.bin_hi && lane < npb) cur_base[pb - it.p0 + lane] = ncur;
         }
         // the window's buckets into the partial trace; the spill bucket stays
-        // in LDS for the next window
+        // in the registers for the next window
         if (active) {
             if (PREC == 0 || first_flush) {
                 for (int j = 0; j < nw; ++j) store(j, j == 0 && b0_stored);

@@ -387,22 +387,22 @@ int default_window(const tg_context* c, int n_bins)
 size_t cell_bytes(int prec) { return prec == TG_FP64 ? sizeof(double) : sizeof(float); }
 
 // LDS of one wave: window x 6 components x 64 lanes, plus the FP64
-// wave-uniform walk's record chunks; the per-lane walk's windows have one
-// more bucket, the spill bucket (run_item).
+// wave-uniform walk's record chunks (the per-lane walk's spill bucket is held
+// in registers, run_item).
 size_t stage_bytes(int prec, bool lanes)
 {
     if (lanes) return 0;  // the per-lane walk reads records straight from global memory
     return (prec == TG_FP64 ? SourceOf<0>::lds_doubles : SourceOf<1>::lds_doubles) * sizeof(double);
 }
 
-size_t lds_per_wave(int prec, bool lanes, bool spill, int window)
+size_t lds_per_wave(int prec, bool lanes, int window)
 {
-    return (size_t)(window + (spill ? 1 : 0)) * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, lanes);
+    return (size_t)window * 6 * kLanes * cell_bytes(prec) + stage_bytes(prec, lanes);
 }
 
-int max_window_for_lds(const tg_context* c, int prec, bool lanes, bool spill)
+int max_window_for_lds(const tg_context* c, int prec, bool lanes)
 {
-    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec))) - (spill ? 1 : 0);
+    return (int)((c->max_lds - stage_bytes(prec, lanes)) / (6 * kLanes * cell_bytes(prec)));
 }
 
 
@@ -479,7 +479,7 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     plan.screen_ids = ids;
     plan.lanes = lane_walk(c, tun);
     plan.spill = spill_on(c, tun);
-    int wmax_lds = max_window_for_lds(c, prec, plan.lanes, plan.spill);
+    int wmax_lds = max_window_for_lds(c, prec, plan.lanes);
     size_t part_off = 0;
     for (int id : ids) {
         ScreenHost& s = c->screens[id];
@@ -510,7 +510,7 @@ int make_plan(tg_context* c, int prec, const std::vector<int>& ids, int p0, int 
     // launch short; a group seeds its cursors by search instead of walking
     // from step 0.
     long long base_items = total_tiles * n_chunks;
-    const size_t lds_wave = lds_per_wave(prec, plan.lanes, plan.spill, plan.max_window);
+    const size_t lds_wave = lds_per_wave(prec, plan.lanes, plan.max_window);
     long long slots = (long long)std::max(c->n_cu, 1) * (long long)std::max<size_t>(1, c->max_lds / lds_wave);
     int groups = (int)std::max<long long>(1, std::min<long long>(64, (16 * slots + base_items - 1) / base_items));
     if (tun && tun->window_groups > 0) groups = tun->window_groups;
@@ -1407,7 +1407,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     // how long the launch's tail is); only the items' particle count is
     // smaller.
     const int cands[] = {2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16};
-    int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr), spill_on(c, nullptr));
+    int wmax = max_window_for_lds(c, precision, lane_walk(c, nullptr));
     double pairs_per_particle = 0;
     for (const ScreenHost& s : c->screens) pairs_per_particle += (double)s.n_pix * (c->n_points - 1);
     const double want = 1e11 / std::max(1.0, pairs_per_particle);
@@ -1496,9 +1496,8 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
         // granularity against the search each group costs), the particle
         // chunk (items per tile against partial traces to reduce), each halved
         // and doubled around the default, and the window without its spill
-        // bucket (one bucket less of LDS per wave, so more waves, against
-        // walking every window's crossing steps twice), at the same window and
-        // one wider.  The fastest shape and the default are then timed once
+        // bucket (six registers per lane fewer, against walking every
+        // window's crossing steps twice), at the same window and one wider.  The fastest shape and the default are then timed once
         // more each, and the shape replaces the default only when its fastest
         // trial is more than 1 % faster, so timing noise does not move it.
         if (best_full.chunk > 0) {
@@ -1509,7 +1508,7 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
             float shape_ms = base_ms;
             size_t per_chunk = 0;
             for (int id : ids) per_chunk += (size_t)c->screens[id].n_pix * c->screens[id].n_bins * 6;
-            const int spill_w1 = std::min(best_w + 1, std::min(max_bins, max_window_for_lds(c, precision, true, false)));
+            const int spill_w1 = std::min(best_w + 1, std::min(max_bins, max_window_for_lds(c, precision, true)));
             const int shapes[6][4] = {{best_w, g0 / 2, ch0, 0}, {best_w, g0 * 2, ch0, 0}, {best_w, g0, ch0 * 2, 0},
                                       {best_w, g0, ch0 / 2, 0}, {best_w, g0, ch0, 2},     {spill_w1, g0, ch0, 2}};
             long long pick_nch = 0;
