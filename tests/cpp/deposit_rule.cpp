@@ -20,7 +20,7 @@
 // bucket (a step ending in the next window's first bucket is deposited whole,
 // its share of that bucket carried over).  The traces must equal the
 // oracle's deposit of every step within 1e-14 of the largest trace value
-// (FP64 weights), or within 2e-6 of the largest end field per step touching a
+// (FP64 weights: the FP64 kernel's doubled, select-free step_common_x2), or within 2e-6 of the largest end field per step touching a
 // bucket (bucket units, FP32).  A dropped held-back share, a bucket sum added
 // twice or to the wrong bucket, or a lost spill bucket fails by the size of a
 // step's contribution.
@@ -64,7 +64,7 @@ struct CellAcc {
     int b0;
     const F* f1;
     const F* f2;
-    double scale;   // 2 for bucket units: the mixed window holds twice the sums
+    double scale;   // 2: the kernels' windows hold twice the sums
     void operator()(int j, double a, double b) const
     {
         for (int c = 0; c < 6; c++) cells[6 * (j - b0) + c] += (F)(scale * a) * f1[c] + (F)(scale * b) * f2[c];
@@ -109,12 +109,12 @@ static void lane_walk(int NP, const int* idx, const double* ts, const float* u, 
                 t = tg::step_common_units<F>(RcpF{}, common, idx[k], (F)u[k], idx[k + 1], (F)u[k + 1], i2, b0, b1,
                                              -1, -1);
             else
-                t = tg::step_common<F>(Rcp{}, common, ts[k], idx[k], ts[k + 1], i2, t0, dt, inv, nb, b0, b1, -1,
-                                       -1);
+                t = tg::step_common_x2<F>(Rcp{}, common, ts[k], idx[k], ts[k + 1], i2, t0, dt, inv, b0, b1, -1,
+                                          -1);
             const int ja = idx[k] - b0;
             tg::lane_step(L, add, common, gen || i2 != idx[k], t, ja, (unsigned)ja < (unsigned)nw, f1, f2);
             if (gen) {
-                CellAcc<F> acc{cells.data(), b0, f1, f2, units ? 2.0 : 1.0};
+                CellAcc<F> acc{cells.data(), b0, f1, f2, 2.0};
                 const int hi = b1 + ((spill && i2 == b1) ? 1 : 0);
                 if (units) {
                     const double q1 = (double)idx[k] + (double)u[k], q2 = (double)idx[k + 1] + (double)u[k + 1];
@@ -141,7 +141,7 @@ static void lane_walk(int NP, const int* idx, const double* ts, const float* u, 
             }
         }
         for (int j = 0; j < nw; j++)
-            for (int c = 0; c < 6; c++) trace[6 * (b0 + j) + c] += (units ? 0.5 : 1.0) * (double)cells[6 * j + c];
+            for (int c = 0; c < 6; c++) trace[6 * (b0 + j) + c] += 0.5 * (double)cells[6 * j + c];
         carry = spill;
     }
 }

@@ -62,6 +62,8 @@ __global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict
     double bg2 = ux[g] * ux[g] + uy[g] * uy[g] + uz[g] * uz[g];
     double gamma2 = bg2 + 1.0;
     double gamma = __builtin_sqrt(gamma2);
+    const double scale = charge[p] * kElementaryCharge / (4.0 * kPi * kEpsNull);  // particle.cpp:588
+    const double sa = scale / kSpeedOfLight;
     RecF64 r;
     r.t = t[g];
     r.x = x[g];
@@ -70,11 +72,11 @@ __global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict
     r.bx = ux[g] / gamma;
     r.by = uy[g] / gamma;
     r.bz = uz[g] / gamma;
-    r.bpx = ax[g] / gamma;
-    r.bpy = ay[g] / gamma;
-    r.bpz = az[g] / gamma;
-    r.g2inv = 1.0 / gamma2;
-    r.scale = charge[p] * kElementaryCharge / (4.0 * kPi * kEpsNull);  // particle.cpp:588
+    r.apx = ax[g] / gamma * sa;
+    r.apy = ay[g] / gamma * sa;
+    r.apz = az[g] / gamma * sa;
+    r.sv = scale / gamma2;
+    r.pad = 0.0;
     store_rec(out, n_points, p, g - p * (size_t)n_points, r);
 }
 
@@ -151,7 +153,13 @@ __device__ __forceinline__ void timing_exact(double t, double dx, double dy, dou
 // FP64 Lienard-Wiechert field (src/particle.cpp:570-596).  n = RVec/R is
 // formed bit-identically (Markstein with RN(1/R)) and 1 - n.beta in the
 // reference's summation order, because that difference cancels ~4 digits.
-// The acceleration term uses a x (b x c) = b (a.c) - c (a.b).
+// The acceleration term uses a x (b x c) = b (a.c) - c (a.b), so with
+// ap = beta' scale/c and sv = scale/gamma^2 from the record
+//   E = [(n - beta)(sv/R + n.ap) - ap n.(n - beta)] / (R kappa^3).
+// n.(n - beta) is kept as computed rather than replaced by kappa: |n|^2
+// differs from 1 by an ulp, which is ~1e-12 of kappa, and the reference's
+// cross products see that n.  The point returns c B = n x E (the window's
+// flush applies 1/c).
 __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, double Pz, double t0,
                                          double dt, double inv_dt, Point& o)
 {
@@ -169,24 +177,22 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
         double nb = r.bx * nx + r.by * ny + r.bz * nz;
         kappa = 1.0 - nb;
     }
-    double mx = nx - r.bx, my = ny - r.by, mz = nz - r.bz;   // n - beta
-    double k3 = kappa * kappa * kappa;
-    double irk3 = rcp_amp(R * k3);                            // 1/(R kappa^3)
-    double cv = r.g2inv * invR * irk3;                        // 1/(R^2 kappa^3 gamma^2)
-    double ca = irk3 * kInvSpeedOfLight;                      // 1/(R kappa^3 c)
-    double nbp = nx * r.bpx + ny * r.bpy + nz * r.bpz;        // n . beta'
-    double nm = nx * mx + ny * my + nz * mz;                  // n . (n - beta)
-    double c1 = (cv + ca * nbp) * r.scale;
-    double c2 = ca * nm * r.scale;
-    double ex = mx * c1 - r.bpx * c2;
-    double ey = my * c1 - r.bpy * c2;
-    double ez = mz * c1 - r.bpz * c2;
+    const double mx = nx - r.bx, my = ny - r.by, mz = nz - r.bz;   // n - beta
+    const double k3 = kappa * kappa * kappa;
+    const double irk3 = rcp_amp(R * k3);                          // 1/(R kappa^3)
+    const double nap = nx * r.apx + ny * r.apy + nz * r.apz;      // n . beta' scale/c
+    const double nm = nx * mx + ny * my + nz * mz;                // n . (n - beta)
+    const double c1 = irk3 * __builtin_fma(r.sv, invR, nap);
+    const double c2 = irk3 * nm;
+    const double ex = mx * c1 - r.apx * c2;
+    const double ey = my * c1 - r.apy * c2;
+    const double ez = mz * c1 - r.apz * c2;
     o.f[0] = ex;
     o.f[1] = ey;
     o.f[2] = ez;
-    o.f[3] = (ny * ez - nz * ey) * kInvSpeedOfLight;           // B = n x E / c
-    o.f[4] = (nz * ex - nx * ez) * kInvSpeedOfLight;
-    o.f[5] = (nx * ey - ny * ex) * kInvSpeedOfLight;
+    o.f[3] = ny * ez - nz * ey;   // c B = n x E
+    o.f[4] = nz * ex - nx * ez;
+    o.f[5] = nx * ey - ny * ex;
 }
 
 // Retarded time and bucket position of the mixed kernel, also returning 1/R:
@@ -331,10 +337,10 @@ struct Window {
     }
 };
 
-// The mixed kernel's FP32 window holds twice the bucket sums (its common
-// weights come doubled from step_common_units), halved when it is flushed.
+// Both windows hold twice the bucket sums (the common weights come doubled
+// from step_common_units and step_common_x2), halved when they are flushed.
 template <class F>
-constexpr double kCellScale = sizeof(F) == 4 ? 2.0 : 1.0;
+constexpr double kCellScale = 2.0;
 
 // Accumulator for step_general: absolute bucket j -> window cell j - b0.
 template <class F>
@@ -392,6 +398,7 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 // piece (or, past it, zeros from the resource), which only stopped lanes
 // evaluate.
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 constexpr int kBufferWord3 = 0x00020000;   // raw 32-bit data format, gfx9 buffer resource word 3
 
 __device__ __forceinline__ double dbl(unsigned lo, unsigned hi)
@@ -438,7 +445,9 @@ __device__ __forceinline__ RecMixed RecBuffer<RecMixed>::load(int k) const
 template <>
 __device__ __forceinline__ RecF64 RecBuffer<RecF64>::load(int k) const
 {
-    const u32x4 a = part(k, 0), b = part(k, 1), c = part(k, 2), d = part(k, 3), e = part(k, 4), f = part(k, 5);
+    const u32x4 a = part(k, 0), b = part(k, 1), c = part(k, 2), d = part(k, 3), e = part(k, 4);
+    // the last piece holds sv and the padding: its first 8 B only
+    const u32x2 f = __builtin_amdgcn_raw_buffer_load_b64(rsrc, 16 * k, stride * 5, 0);
     RecF64 r;
     r.t = dbl(a.x, a.y);
     r.x = dbl(a.z, a.w);
@@ -447,11 +456,11 @@ __device__ __forceinline__ RecF64 RecBuffer<RecF64>::load(int k) const
     r.bx = dbl(c.x, c.y);
     r.by = dbl(c.z, c.w);
     r.bz = dbl(d.x, d.y);
-    r.bpx = dbl(d.z, d.w);
-    r.bpy = dbl(e.x, e.y);
-    r.bpz = dbl(e.z, e.w);
-    r.g2inv = dbl(f.x, f.y);
-    r.scale = dbl(f.z, f.w);
+    r.apx = dbl(d.z, d.w);
+    r.apy = dbl(e.x, e.y);
+    r.apz = dbl(e.z, e.w);
+    r.sv = dbl(f.x, f.y);
+    r.pad = 0.0;
     return r;
 }
 
@@ -472,8 +481,8 @@ __device__ __forceinline__ void deposit_window(const Window<double>& win, bool a
     const bool case_a = p1.ts <= tmin;
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
-    const TwoBuckets<double> t = step_common<double>(RcpAmp{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
-                                                     inv_dt, n_bins, b0, b1, -1, -1);
+    const TwoBuckets<double> t = step_common_x2<double>(RcpAmp{}, valid && !gen, p1.ts, p1.idx, p2.ts, i2, t0, dt,
+                                                        inv_dt, b0, b1, -1, -1);
     win.add2(t.ja, t.aa, t.ba, t.jb, t.ab, t.bb, p1.f, p2.f);
     if (__ballot(gen)) {
         if (gen) {
@@ -536,8 +545,8 @@ __device__ __forceinline__ void deposit_lane(const Window<double>& win, LaneAcc<
     const int i2 = p2.idx > n_bins ? n_bins : p2.idx;
     const bool gen = valid && step_is_general(case_a, p1.idx, i2);
     const bool common = valid && !gen;
-    const TwoBuckets<double> t = step_common<double>(RcpAmp{}, common, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt,
-                                                     n_bins, b0, b1, -1, -1);
+    const TwoBuckets<double> t = step_common_x2<double>(RcpAmp{}, common, p1.ts, p1.idx, p2.ts, i2, t0, dt, inv_dt,
+                                                        b0, b1, -1, -1);
     const int ja = p1.idx - b0;
     lane_step(L, WindowAdd<double>{win}, common, walking && (gen || i2 != p1.idx), t, ja,
               (unsigned)ja < (unsigned)(b1 - b0), p1.f, p2.f);
@@ -1053,12 +1062,9 @@ __device__ __forceinline__ void run_item(const WorkItem& it, const ScreenDev* __
         bool dirty = carry;    // the window holds terms not yet in the partial trace
         // estimated terms per cell since the last flush, times nw
         unsigned terms = carry ? carry_terms : 0;
-        // component c of a window cell as an FP64 trace value (mixed cells
-        // hold twice the sums, and c B for the magnetic components)
-        auto value = [&](F raw, int c) {
-            const double v = (double)raw;
-            return PREC == 1 ? v * (c >= 3 ? 0.5 * kInvSpeedOfLight : 0.5) : v;
-        };
+        // component c of a window cell as an FP64 trace value (cells hold
+        // twice the sums, and c B for the magnetic components)
+        auto value = [&](F raw, int c) { return (double)raw * (c >= 3 ? 0.5 * kInvSpeedOfLight : 0.5); };
         // bucket j < nw (LDS), or j = nw (the spill registers), into out
         auto store = [&](int j, bool add) {
             add = add || it.chunk == 0;

@@ -71,6 +71,39 @@ TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, 
     return o;
 }
 
+// step_common for the FP64 radiation kernel: twice the weights (its window
+// holds twice the bucket sums and halves them when it is flushed: a power of
+// two, so every rounding is the same) and no selects on the weights.  A
+// single-bucket step has its "edge" at e = ts2, so db = 0 and lb = 0 exactly
+// and la = dts RN(1/dts), within an ulp of the 1 of step_common; a
+// zero-length step adds zeros (its length is raised to 1e-300 before the
+// reciprocal; retarded times differ by at least an ulp, ~1e-24 s, or not at
+// all).  [b0, b1) lies within the trace.
+template <class F, class Rcp>
+TG_HD inline TwoBuckets<F> step_common_x2(const Rcp& rcp, bool common, double ts1, int idx1, double ts2, int i2,
+                                          double t0, double dt, double inv_dt, int b0, int b1, int none_a,
+                                          int none_b)
+{
+#pragma clang fp contract(off)
+    TwoBuckets<F> o;
+    const bool single = i2 == idx1;
+    const double e = single ? ts2 : t0 + (idx1 + 0.5) * dt;     // particle.cpp:841 t_end
+    const F da = (F)(e - ts1), db = (F)(ts2 - e);               // exact time differences
+    const F len = da + db;
+    const F idts = rcp(len > (F)1e-300 ? len : (F)1e-300);
+    const F la = da * idts, lb = db * idts;
+    const F fid = (F)inv_dt;
+    const F ha = da * fid, hb = db * fid;
+    o.aa = ha * ((F)1 + lb);
+    o.ba = ha * la;
+    o.ab = hb * lb;
+    o.bb = hb * (la + (F)1);
+    const unsigned span = (unsigned)(b1 - b0);
+    o.ja = (common && (unsigned)(idx1 - b0) < span) ? idx1 - b0 : none_a;
+    o.jb = (common && !single && (unsigned)(i2 - b0) < span) ? i2 - b0 : none_b;
+    return o;
+}
+
 // step_common in bucket units, for the mixed kernel: a point sits at bucket
 // position q = (ts - t0)/dt = idx + u, u in [-1/2, 1/2] its offset from the
 // centre of its bucket idx.  Positions are taken relative to the centre of

@@ -28,17 +28,18 @@ constexpr double kInvSpeedOfLight = 1.0 / kSpeedOfLight;  // RN(1/c), folded by 
 
 constexpr int kLanes = 64;  // one wavefront = one pixel tile
 
-// Packed trajectory point for the FP64 kernel: 12 doubles = 96 B, read per
+// Packed trajectory point for the FP64 kernel: 11 doubles in 96 B, read per
 // lane (vector loads) by the per-lane walk, or staged through LDS for the
-// wave-uniform walk.  beta = u/gamma and
-// beta' = a/gamma are hoisted out of the pixel loop; both are computed exactly
-// as src/particle.cpp:575-582 computes them.
+// wave-uniform walk.  beta = u/gamma and beta' = a/gamma are hoisted out of
+// the pixel loop, computed exactly as src/particle.cpp:575-582 computes them;
+// the charge scale q e/(4 pi eps0) of src/particle.cpp:588 is folded into
+// the two amplitude factors of the field (eval_f64).
 struct alignas(32) RecF64 {
     double t, x, y, z;
     double bx, by, bz;
-    double bpx, bpy, bpz;
-    double g2inv;   // 1/gamma^2
-    double scale;   // q e / (4 pi eps0)
+    double apx, apy, apz;   // beta' * scale / c
+    double sv;              // scale / gamma^2
+    double pad;
 };
 
 // Packed trajectory point for the mixed kernel: 64 B, read per lane with
@@ -162,14 +163,23 @@ __device__ __forceinline__ double sqrt_rn(double x)
     return __builtin_fma(d, h, g);
 }
 
-// round() of the reference (half away from zero), clamped into int range;
-// the clamp only touches values far outside every trace, where the reference
-// never evaluates the index.
+// v_cvt_i32_f64: saturates out-of-range values (infinities included) to the
+// int range, NaN to 0 (CDNA ISA, float-to-int conversions).  Written as the
+// instruction because a C++ conversion of an out-of-range value is undefined.
+__device__ __forceinline__ int cvt_sat_i32(double r)
+{
+    int i;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(i) : "v"(r));
+    return i;
+}
+
+// round() of the reference (half away from zero), clamped to +-1e9 (one
+// v_med3_i32 after the saturating conversion); the clamp only touches values
+// far outside every trace, where the reference never evaluates the index.
 __device__ __forceinline__ int round_index(double q)
 {
-    double r = __builtin_round(q);
-    r = __builtin_fmin(__builtin_fmax(r, -1.0e9), 1.0e9);
-    return (int)r;
+    const int i = cvt_sat_i32(__builtin_round(q));
+    return min(max(i, -1000000000), 1000000000);
 }
 
 // Nearest integer (half to even) of q, clamped into int range: the mixed
