@@ -196,18 +196,21 @@ __device__ __forceinline__ void eval_f64(const RecF64& r, double Px, double Py, 
 }
 
 // Retarded time and bucket position of the mixed kernel, also returning 1/R:
-// 1/sqrt from the FP32 hardware estimate and one FP64 Newton step (~2^-46
-// relative).  Every FMA is explicit, so every caller gets the same doubles.
-// The position q = (ts - t0)/dt is returned as its nearest bucket idx (half
-// to even, which differs from the reference's half-away rounding only for a
-// retarded time exactly on a bucket edge; clamped to +-1e9) and the offset
-// u = q - idx, in [-1/2, 1/2] inside the clamp.
+// 1/sqrt from the FP32 hardware estimate fy (~1 ulp of FP32, which the FP32
+// amplitudes use as it is) and one FP64 Newton step, y (~2^-46 relative).
+// Every FMA is explicit, so every caller gets the same doubles.  The position
+// q = (ts - t0)/dt is returned as its nearest bucket idx (half to even, which
+// differs from the reference's half-away rounding only for a retarded time
+// exactly on a bucket edge; clamped to +-1e9) and the offset u = q - rint(q),
+// in [-1/2, 1/2].  Beyond the clamp (|q| > 1e9 buckets, a point a billion
+// buckets away from the trace) idx + u is not q.
 __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, double dz, double t0,
-                                             double inv_dt, double& y, int& idx, float& u)
+                                             double inv_dt, double& y, float& fy, int& idx, float& u)
 {
 #pragma clang fp contract(off)
     const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
-    y = (double)__builtin_amdgcn_rsqf((float)r2);
+    fy = __builtin_amdgcn_rsqf((float)r2);
+    y = (double)fy;
     const double h = r2 * y;
     const double e = __builtin_fma(-h, y, 1.0);
     y = __builtin_fma(0.5 * y, e, y);
@@ -215,8 +218,8 @@ __device__ __forceinline__ void timing_mixed(double t, double dx, double dy, dou
     // q = (t + R/c - t0)/dt as R (1/(c dt)) + (t/dt - t0/dt): both terms and
     // q keep ~1e-9 of a bucket (t/dt ~ 1e7)
     const double q = __builtin_fma(R, kInvSpeedOfLight * inv_dt, __builtin_fma(t, inv_dt, -(t0 * inv_dt)));
-    const double r = __builtin_fmin(__builtin_fmax(__builtin_rint(q), -1.0e9), 1.0e9);
-    idx = (int)r;
+    const double r = __builtin_rint(q);
+    idx = min(max(cvt_sat_i32(r), -1000000000), 1000000000);
     u = (float)(q - r);
 }
 
@@ -227,9 +230,9 @@ __device__ __forceinline__ void eval_mixed(const RecMixed& r, double Px, double 
 {
     const double dx = Px - r.x, dy = Py - r.y, dz = Pz - r.z;
     double y;
-    timing_mixed(r.t, dx, dy, dz, t0, inv_dt, y, o.idx, o.u);
+    float fy;
+    timing_mixed(r.t, dx, dy, dz, t0, inv_dt, y, fy, o.idx, o.u);
     const double nz = dz * y;
-    const float fy = (float)y;
     const float nx = (float)dx * fy, ny = (float)dy * fy, fnz = (float)nz;
     const float kappa = (float)__builtin_fma(-nz, r.bz, 1.0) - (nx * r.bx + ny * r.by);
     const float mx = nx - r.bx, my = ny - r.by, mz = (float)(nz - r.bz);
@@ -790,8 +793,8 @@ __device__ __forceinline__ int point_index(const RecBuffer<typename RecOf<PREC>:
         timing_exact(t, Px - x, Py - y, Pz - z, t0, dt, inv_dt, R, ts, idx);
     } else {
         double yr;
-        float u;
-        timing_mixed(t, Px - x, Py - y, Pz - z, t0, inv_dt, yr, idx, u);
+        float fy, u;
+        timing_mixed(t, Px - x, Py - y, Pz - z, t0, inv_dt, yr, fy, idx, u);
     }
     return idx;
 }

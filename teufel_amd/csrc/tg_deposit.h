@@ -78,7 +78,9 @@ TG_HD inline TwoBuckets<F> step_common(const Rcp& rcp, bool common, double ts1, 
 // and la = dts RN(1/dts), within an ulp of the 1 of step_common; a
 // zero-length step adds zeros (its length is raised to 1e-300 before the
 // reciprocal; retarded times differ by at least an ulp, ~1e-24 s, or not at
-// all).  [b0, b1) lies within the trace.
+// all).  A step that is not common gets zero weights (its time differences
+// are taken as zero), as lane_step requires.  [b0, b1) lies within the
+// trace.
 template <class F, class Rcp>
 TG_HD inline TwoBuckets<F> step_common_x2(const Rcp& rcp, bool common, double ts1, int idx1, double ts2, int i2,
                                           double t0, double dt, double inv_dt, int b0, int b1, int none_a,
@@ -88,7 +90,7 @@ TG_HD inline TwoBuckets<F> step_common_x2(const Rcp& rcp, bool common, double ts
     TwoBuckets<F> o;
     const bool single = i2 == idx1;
     const double e = single ? ts2 : t0 + (idx1 + 0.5) * dt;     // particle.cpp:841 t_end
-    const F da = (F)(e - ts1), db = (F)(ts2 - e);               // exact time differences
+    const F da = common ? (F)(e - ts1) : (F)0, db = common ? (F)(ts2 - e) : (F)0;   // exact time differences
     const F len = da + db;
     const F idts = rcp(len > (F)1e-300 ? len : (F)1e-300);
     const F la = da * idts, lb = db * idts;
@@ -113,8 +115,10 @@ TG_HD inline TwoBuckets<F> step_common_x2(const Rcp& rcp, bool common, double ts
 //   (e - ts1)/dt = e - u1,  (ts2 - e)/dt = x2 - e,  dts/dt = x2 - u1,
 // and no weight needs a select: a single-bucket step gets lb = 0 from
 // x2 - e = 0.  A zero-length step adds zeros (its length is raised to
-// 1e-30 before the reciprocal).  idx2 is the unclamped index of ts2
-// (i2 = min(idx2, n_bins)); [b0, b1) lies within the trace.  The weights are
+// 1e-30 before the reciprocal), and so does a step that is not common (its
+// differences are taken as zero), as lane_step requires.  idx2 is the
+// unclamped index of ts2 (i2 = min(idx2, n_bins)); [b0, b1) lies within the
+// trace.  The weights are
 // twice those of step_common (the mixed kernel's window holds twice the
 // bucket sums and halves them when it is flushed: a power of two, so every
 // rounding is the same, and the weights need no factor 1/2).
@@ -126,8 +130,8 @@ TG_HD inline TwoBuckets<F> step_common_units(const Rcp& rcp, bool common, int id
     const bool single = i2 == idx1;
     const F x2 = (F)(idx2 - idx1) + u2;
     const F e = single ? x2 : (F)0.5;
-    const F da = e - u1, db = x2 - e;
-    const F len = x2 - u1;
+    const F da = common ? e - u1 : (F)0, db = common ? x2 - e : (F)0;
+    const F len = da + db;
     const F idts = rcp(len > (F)1e-30 ? len : (F)1e-30);
     const F la = da * idts, lb = db * idts;
     o.aa = da * ((F)1 + lb);
@@ -170,8 +174,9 @@ TG_HD inline void lane_begin(LaneAcc<F>& L)
 }
 
 // Step p1 -> p2 of a lane that walks it.  common: valid and not general,
-// with weights t; any other step adds nothing here (a general one goes to
-// step_general), and its end point starts the next bucket sum from zero.
+// with weights t (step_common_x2 or step_common_units, which give any other
+// step zero weights); any other step adds nothing here (a general one goes
+// to step_general), and its end point starts the next bucket sum from zero.
 // leaves: the step does not end in bucket idx1 (crossing or general); ja: the
 // window-relative index of bucket idx1, in_win: whether it lies in the
 // window.  add(j, c, v) adds v to component c of window bucket j.
@@ -179,17 +184,16 @@ template <class F, class Add>
 TG_HD inline void lane_step(LaneAcc<F>& L, const Add& add, bool common, bool leaves, const TwoBuckets<F>& t, int ja,
                             bool in_win, const F* f1, const F* f2)
 {
-    const F aa = common ? t.aa : (F)0, ba = common ? t.ba : (F)0;
-    const F wa = L.wc + aa;
+    (void)common;
+    const F wa = L.wc + t.aa;
     for (int c = 0; c < 6; ++c) L.acc[c] = fma_f(wa, f1[c], L.acc[c]);
     if (leaves) {
         if (in_win)
-            for (int c = 0; c < 6; ++c) add(ja, c, fma_f(ba, f2[c], L.acc[c]));
-        const F ab = common ? t.ab : (F)0;
-        for (int c = 0; c < 6; ++c) L.acc[c] = ab * f1[c];
-        L.wc = common ? t.bb : (F)0;
+            for (int c = 0; c < 6; ++c) add(ja, c, fma_f(t.ba, f2[c], L.acc[c]));
+        for (int c = 0; c < 6; ++c) L.acc[c] = t.ab * f1[c];
+        L.wc = t.bb;
     } else {
-        L.wc = ba;
+        L.wc = t.ba;
     }
 }
 
