@@ -44,6 +44,11 @@
 //   host_api_test screenread <screen.h5> <out.dump>
 //       reads a ScreenObserver HDF5 file and writes its cells and traces as a
 //       plain dump (positions, cell t0, samples) for the Python tests
+//   host_api_test pointsdds <out.sdds>
+//       a PointObserver<Bunch> with known field values written by
+//       generateOutput(): the ASCII SDDS file of src/point_observer.cpp:120-182,
+//       which the Python test parses and checks value for value (no GPU
+//       needed)
 //   host_api_test snaph5 <out.h5>
 //       a SnapshotObserver<Bunch> with known field values written by
 //       generateOutput() (the HDF5 file of src/snapshot.cpp:226-392) and read
@@ -544,6 +549,22 @@ int run_screenh5(const char* out)
     return errors;
 }
 
+// PointObserver SDDS file written by generateOutput; the field of sample it,
+// component c, is it * 1000 + c * 0.001 - 1/3 (1/3 rounded: a value that
+// needs all 17 digits)
+int run_pointsdds(const char* out)
+{
+    const unsigned nots = 37;
+    PointObserver<Bunch> obs(out, Vector(0.01, -0.02, 10.0), 3.3e-8, 2.5e-15, nots);
+    auto value = [](unsigned it, int c) { return it * 1000.0 + c * 0.001 - 1.0 / 3.0; };
+    for (unsigned it = 0; it < nots; it++)
+        obs.setField(it, ElMagField(Vector(value(it, 0), value(it, 1), value(it, 2)),
+                                    Vector(value(it, 3), value(it, 4), value(it, 5))));
+    obs.generateOutput();
+    std::printf("pointsdds: %u samples written\n", nots);
+    return 0;
+}
+
 // SnapshotObserver HDF5 file written by generateOutput and read back
 int run_snaph5(const char* out)
 {
@@ -623,6 +644,8 @@ int main(int argc, char** argv)
             return run_screenh5(argv[2]);
         } else if (mode == "screenread" && argc >= 4) {
             return run_screenread(argv[2], argv[3]);
+        } else if (mode == "pointsdds" && argc >= 3) {
+            return run_pointsdds(argv[2]);
         } else if (mode == "snaph5" && argc >= 3) {
             return run_snaph5(argv[2]);
         } else if (mode == "diffraction" && argc >= 5) {

@@ -151,3 +151,58 @@ def test_corrupt_screen_observer_files(program, tmp_path):
     outcomes = _corrupt_runs(program, tmp_path, src, "screenread", 30, 80, 11)
     print("screenread on corrupt files:", outcomes)
     assert outcomes[IOEXCEPTION] > 0
+
+
+def _read_sdds_ascii(path):
+    """A reader of version-1 ASCII SDDS files, written from the protocol (the
+    "SDDS1" line, &parameter / &column / &data namelists up to &end, then per
+    page: one line per parameter value, the row count, one line per row;
+    lines starting with '!' are comments).  Returns the definitions, the
+    parameter values and the columns of the first page."""
+    import shlex
+    lines = open(path).read().splitlines()
+    assert lines[0] == "SDDS1"
+    params, cols, data = [], [], None
+    i = 1
+    while data is None:
+        ln = lines[i].strip()
+        i += 1
+        assert ln.startswith("&") and ln.endswith("&end"), ln
+        kind, body = ln[1:].split(None, 1)
+        fields = {}
+        for item in shlex.split(body.replace(",", " ")):
+            if item == "&end":
+                continue
+            k, _, v = item.partition("=")
+            fields[k] = v
+        {"parameter": params, "column": cols}.get(kind, []).append(fields)
+        if kind == "data":
+            data = fields
+    assert data["mode"] == "ascii"
+    body = [ln for ln in lines[i:] if not ln.startswith("!")]
+    pvals = {p["name"]: (int(v) if p["type"] == "long" else float(v)) for p, v in zip(params, body)}
+    nrows = int(body[len(params)])
+    rows = np.array([[float(x) for x in ln.split()] for ln in body[len(params) + 1:len(params) + 1 + nrows]])
+    assert len(body) == len(params) + 1 + nrows
+    return params, cols, pvals, {c["name"]: rows[:, j] for j, c in enumerate(cols)}
+
+
+def test_point_observer_sdds_file(program, tmp_path):
+    """PointObserver::generateOutput writes the reference's ASCII SDDS file
+    (src/point_observer.cpp:120-182, read by scripts/plot_PointObservation.py):
+    its parameters, columns (names, units, types) and every value, exactly."""
+    out = str(tmp_path / "point.sdds")
+    rc, rep, log = run(program, "pointsdds", out)
+    assert rc == 0 and not rep, log[-3000:]
+    params, cols, pv, cv = _read_sdds_ascii(out)
+    assert [(p["name"], p["type"]) for p in params] == [("NumberTimeSteps", "long"), ("t0", "double"),
+                                                        ("dt", "double")]
+    assert [(c["name"], c["units"], c["type"]) for c in cols] == [
+        ("t", "s", "double"), ("Ex", "V/m", "double"), ("Ey", "V/m", "double"), ("Ez", "V/m", "double"),
+        ("Bx", "T", "double"), ("By", "T", "double"), ("Bz", "T", "double")]
+    nots, t0, dt = 37, 3.3e-8, 2.5e-15
+    assert pv == {"NumberTimeSteps": nots, "t0": t0, "dt": dt}
+    it = np.arange(nots)
+    assert np.array_equal(cv["t"], t0 + it * dt)
+    for c, name in enumerate(("Ex", "Ey", "Ez", "Bx", "By", "Bz")):
+        assert np.array_equal(cv[name], it * 1000.0 + c * 0.001 - 1.0 / 3.0)

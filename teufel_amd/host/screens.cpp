@@ -417,6 +417,47 @@ void PointObserver<SourceT>::fromBuffer(double* buffer, std::size_t size)
     buffer_to_traces(buffer, size, getBufferSize(), std::vector<FieldTrace*>(1, trace), "PointObserver");
 }
 
+// ASCII SDDS, version 1 of the protocol (M. Borland, "A Self-Describing File
+// Protocol for Simulation Integration and Shared Postprocessors", PAC 1995):
+// the "SDDS1" line, the &parameter and &column namelists, "&data
+// mode=ascii", then one page: each parameter's value on a line of its own,
+// the row count, and one line per row.  The layout is the one
+// src/point_observer.cpp:120-182 makes with the SDDS library (names,
+// symbols, units, descriptions and types of its definitions); doubles are
+// written with 17 significant digits, so every value reads back exactly.
+template <typename SourceT>
+void PointObserver<SourceT>::WriteTimeDomainFieldSDDS()
+{
+    if (FileName.empty()) return;
+    FILE* f = std::fopen(FileName.c_str(), "w");
+    if (!f) throw IOexception("PointObserver::WriteTimeDomainFieldSDDS - cannot open " + FileName);
+    const unsigned int nots = (unsigned int)trace->get_N();
+    const double t0 = trace->get_t0(), dt = trace->get_dt();
+    bool ok = std::fprintf(f, "SDDS1\n") > 0;
+    ok = ok && std::fprintf(f, "&parameter name=NumberTimeSteps, type=long, &end\n") > 0;
+    ok = ok && std::fprintf(f, "&parameter name=t0, type=double, &end\n") > 0;
+    ok = ok && std::fprintf(f, "&parameter name=dt, type=double, &end\n") > 0;
+    const char* col[7][3] = {{"t", "s", "time"},
+                             {"Ex", "V/m", "electric field"},
+                             {"Ey", "V/m", "electric field"},
+                             {"Ez", "V/m", "electric field"},
+                             {"Bx", "T", "magnetic field"},
+                             {"By", "T", "magnetic field"},
+                             {"Bz", "T", "magnetic field"}};
+    for (const auto& c : col)
+        ok = ok && std::fprintf(f, "&column name=%s, symbol=%s, units=%s, description=\"%s\", type=double, &end\n", c[0],
+                                c[0], c[1], c[2]) > 0;
+    ok = ok && std::fprintf(f, "&data mode=ascii, &end\n! page number 1\n%u\n%.17g\n%.17g\n%u\n", nots, t0, dt, nots) > 0;
+    for (unsigned int i = 0; ok && i < nots; i++) {
+        const ElMagField F = trace->get_field((std::size_t)i);
+        const Vector E = F.E(), B = F.B();
+        ok = std::fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", t0 + (double)i * dt, E.x, E.y, E.z, B.x, B.y,
+                          B.z) > 0;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) throw IOexception("PointObserver::WriteTimeDomainFieldSDDS - write error on " + FileName);
+}
+
 template class MeshedScreen<Beam>;
 template class MeshedScreen<Bunch>;
 template class ScreenObserver<Beam>;

@@ -4,9 +4,9 @@
 // integrate() hands all pixels of the observer to the GPU in one batched
 // launch instead of the reference's OpenMP pixel loop (src/mesh.cpp:288-342,
 // src/screen.cpp:112-180, src/point_observer.cpp:62-72).  SourceT is Beam or
-// Bunch.  generateOutput() writes the reference's HDF5 layouts for
-// MeshedScreen (read from a file) and ScreenObserver; PointObserver (SDDS in
-// the reference) and MeshedScreen built from points write a plain binary dump
+// Bunch.  generateOutput() writes the reference's file layouts: HDF5 for
+// MeshedScreen (read from a file) and ScreenObserver, ASCII SDDS for
+// PointObserver; a MeshedScreen built from points writes a plain binary dump
 // (documented in writeDump).
 #pragma once
 
@@ -149,7 +149,11 @@ public:
     std::size_t getBufferSize() override { return 6 * trace->get_N(); }
     double* getBuffer() override;
     void fromBuffer(double* buffer, std::size_t size) override;
-    void generateOutput() override { writeDump(FileName, std::vector<Vector>(1, Pos), std::vector<FieldTrace*>(1, trace)); }
+    // The reference's ASCII SDDS file (src/point_observer.cpp:120-182, read by
+    // scripts/plot_PointObservation.py): parameters NumberTimeSteps, t0, dt;
+    // columns t, Ex, Ey, Ez (V/m), Bx, By, Bz (T), one row per sample.
+    void WriteTimeDomainFieldSDDS();
+    void generateOutput() override { WriteTimeDomainFieldSDDS(); }
 
 private:
     std::string FileName;
