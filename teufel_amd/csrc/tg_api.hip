@@ -190,6 +190,7 @@ int alloc_traj(tg_context* c, int n_particles, int n_points)
         return fail(c, TG_ERR_ARG, "more than 22 million points per particle");
     int rc = free_traj(c);
     if (rc) return rc;
+    c->traj_set = false;   // until the new set is allocated
     size_t n = (size_t)n_particles * (size_t)n_points;
     TG_HIP(c, hipMalloc((void**)&c->d_raw, std::max<size_t>(1, kNumRaw * n + (size_t)n_particles) * sizeof(double)));
     c->n_particles = n_particles;
@@ -1055,17 +1056,27 @@ int tg_add_screen(tg_context* c, int n_pix, const double* obs, const double* t0,
     s.obs.assign(obs, obs + (size_t)3 * n_pix);
     s.t0.assign(t0, t0 + (size_t)n_pix);
     size_t nt = (size_t)n_pix * n_bins * 6;
-    TG_HIP(c, hipMalloc((void**)&s.d_obs, std::max<size_t>(1, (size_t)3 * n_pix) * sizeof(double)));
-    TG_HIP(c, hipMalloc((void**)&s.d_t0, std::max<size_t>(1, (size_t)n_pix) * sizeof(double)));
-    TG_HIP(c, hipMalloc((void**)&s.d_traces, std::max<size_t>(1, nt) * sizeof(double)));
-    if (n_pix) {
-        TG_HIP(c, hipMemcpy(s.d_obs, obs, (size_t)3 * n_pix * sizeof(double), hipMemcpyHostToDevice));
-        TG_HIP(c, hipMemcpy(s.d_t0, t0, (size_t)n_pix * sizeof(double), hipMemcpyHostToDevice));
-    }
-    if (nt) {
-        fill_kernel<<<(unsigned)std::min<size_t>((nt + 255) / 256, 16384), 256, 0, c->stream>>>(s.d_traces, nt, 0.0);
-        TG_HIP(c, hipGetLastError());
-        TG_HIP(c, hipStreamSynchronize(c->stream));
+    // the screen's device arrays, released again if any step of the set-up fails
+    auto setup = [&]() -> int {
+        TG_HIP(c, hipMalloc((void**)&s.d_obs, std::max<size_t>(1, (size_t)3 * n_pix) * sizeof(double)));
+        TG_HIP(c, hipMalloc((void**)&s.d_t0, std::max<size_t>(1, (size_t)n_pix) * sizeof(double)));
+        TG_HIP(c, hipMalloc((void**)&s.d_traces, std::max<size_t>(1, nt) * sizeof(double)));
+        if (n_pix) {
+            TG_HIP(c, hipMemcpy(s.d_obs, obs, (size_t)3 * n_pix * sizeof(double), hipMemcpyHostToDevice));
+            TG_HIP(c, hipMemcpy(s.d_t0, t0, (size_t)n_pix * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (nt) {
+            fill_kernel<<<(unsigned)std::min<size_t>((nt + 255) / 256, 16384), 256, 0, c->stream>>>(s.d_traces, nt,
+                                                                                                      0.0);
+            TG_HIP(c, hipGetLastError());
+            TG_HIP(c, hipStreamSynchronize(c->stream));
+        }
+        return TG_OK;
+    };
+    if (const int rc = setup()) {
+        for (double* d : {s.d_obs, s.d_t0, s.d_traces})
+            if (d) (void)hipFree(d);
+        return rc;
     }
     c->screens.push_back(std::move(s));
     free_graph(c);
