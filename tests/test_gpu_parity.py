@@ -508,3 +508,54 @@ def test_errors_are_reported(ctx):
     # (no particles, so no array is read)
     with pytest.raises(teufel_amd.TgError, match="points per particle"):
         ctx._chk(ctx.lib.tg_set_trajectories(ctx.h, 0, 23_000_000, *([None] * 10), None))
+
+
+def test_unphysical_trajectories_are_rejected():
+    """The walks rely on the retarded time increasing along every trajectory,
+    which holds for any particle slower than light.  A step faster than light
+    or back in time makes integrate fail with TgError in both precisions
+    (the pack kernels check every step); repeated points (zero-length steps)
+    are accepted.  A beam far off the z axis is rejected by TG_MIXED only."""
+    say("unphysical trajectories: faster-than-light and backward steps rejected, repeated points accepted")
+    n, k = 3, 5
+    base = oracle.Trajectories.empty(n, k)
+    base.t[:] = np.arange(k) * 1e-12
+    base.z[:] = np.arange(k) * 2.99e-4            # beta = 0.9973 along z
+    base.uz[:] = 14.0
+    obs, _, _ = W.grid_screen(4, 0.02, 10.0)
+    t0 = np.linalg.norm(obs, axis=1) / W.C_LIGHT - 1e-15
+
+    def variant(**change):
+        tr = oracle.Trajectories(**{nm: getattr(base, nm).copy() for nm in oracle.Trajectories.NAMES},
+                                 charge=base.charge.copy())
+        for (name, p, j), v in change.items():
+            getattr(tr, name)[p, j] = v
+        return tr
+
+    cases = {
+        "superluminal": variant(**{("x", 1, 3): 0.5}),        # a 0.5 m jump within 1 ps
+        "backward": variant(**{("t", 2, 2): 0.5e-12, ("t", 2, 3): 0.4e-12}),
+    }
+    with teufel_amd.RadiationContext(0) as c:
+        sid = c.add_screen(obs, t0, 1e-15, 64)
+        for name, tr in cases.items():
+            c.set_trajectories(tr)
+            for prec in (TG_FP64, TG_MIXED):
+                with pytest.raises(teufel_amd.TgError, match="faster than light"):
+                    c.integrate(prec)
+        # repeated points: a particle resting for a step (dt = 0, dX = 0)
+        rest = variant(**{("t", 0, 4): 3e-12, ("z", 0, 4): 3 * 2.99e-4})
+        c.set_trajectories(rest)
+        c.zero_screen(sid)
+        c.integrate(TG_FP64)
+        ref = oracle.integrate_screen(rest, obs, t0, 1e-15, 64)
+        assert np.abs(ref).max() > 0
+        assert trace_rel_error(c.read_traces(sid), ref) <= FP64_TOL
+        # a beam far off the z axis: FP64 only
+        off = variant()
+        off.ux[:] = 10.0
+        c.set_trajectories(off)
+        c.zero_screen(sid)
+        with pytest.raises(teufel_amd.TgError, match="close to the z axis"):
+            c.integrate(TG_MIXED)
+        c.integrate(TG_FP64)

@@ -34,6 +34,25 @@ namespace tg {
 // Packing: raw ChargedParticle trajectory (t, X, u=beta*gamma, a) -> records.
 // ---------------------------------------------------------------------------
 
+// Trajectory checks of the pack kernels, bits of their flags word.
+constexpr int kFlagOffAxis = 1;       // |beta_x| + |beta_y| > 0.25 (the mixed record)
+constexpr int kFlagSuperluminal = 2;  // a step with |dX| > c dt or dt < 0
+
+// Flags step (point g -> g + 1) of a particle if it is faster than light or
+// goes back in time.  For any particle slower than light the retarded time
+// t + |P - X|/c increases along the trajectory, seen from every point P (its
+// increment is at least dt - |dX|/c); the walks rely on that.  Repeated
+// points (dt = 0, dX = 0) are allowed: they make zero-length steps.
+__device__ __forceinline__ void check_step(size_t g, const double* __restrict__ t, const double* __restrict__ x,
+                                           const double* __restrict__ y, const double* __restrict__ z,
+                                           int* __restrict__ flags)
+{
+    const double dT = t[g + 1] - t[g];
+    const double ex = x[g + 1] - x[g], ey = y[g + 1] - y[g], ez = z[g + 1] - z[g];
+    const double cdt = kSpeedOfLight * dT;
+    if (dT < 0.0 || ex * ex + ey * ey + ez * ez > cdt * cdt) atomicOr(flags, kFlagSuperluminal);
+}
+
 // Record of point k of particle p into the piece-major layout (rec_piece_offset).
 template <class Rec>
 __device__ __forceinline__ void store_rec(char* out, int n_points, size_t p, size_t k, const Rec& r)
@@ -52,12 +71,13 @@ __global__ void pack_f64_kernel(size_t n, int n_points, const double* __restrict
                                 const double* __restrict__ uy, const double* __restrict__ uz,
                                 const double* __restrict__ ax, const double* __restrict__ ay,
                                 const double* __restrict__ az, const double* __restrict__ charge,
-                                char* __restrict__ out)
+                                char* __restrict__ out, int* __restrict__ flags)
 {
 #pragma clang fp contract(off)
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     size_t p = g / (size_t)n_points;
+    if (g - p * (size_t)n_points + 1 < (size_t)n_points) check_step(g, t, x, y, z, flags);
     // src/particle.cpp:575-582: gamma2 = |u|^2 + 1, beta = u/gamma, beta' = a/gamma
     double bg2 = ux[g] * ux[g] + uy[g] * uy[g] + uz[g] * uz[g];
     double gamma2 = bg2 + 1.0;
@@ -86,12 +106,13 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
                                   const double* __restrict__ uy, const double* __restrict__ uz,
                                   const double* __restrict__ ax, const double* __restrict__ ay,
                                   const double* __restrict__ az, const double* __restrict__ charge,
-                                  char* __restrict__ out, int* __restrict__ off_axis)
+                                  char* __restrict__ out, int* __restrict__ flags)
 {
 #pragma clang fp contract(off)
     size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     size_t p = g / (size_t)n_points;
+    if (g - p * (size_t)n_points + 1 < (size_t)n_points) check_step(g, t, x, y, z, flags);
     double bg2 = ux[g] * ux[g] + uy[g] * uy[g] + uz[g] * uz[g];
     double gamma2 = bg2 + 1.0;
     double gamma = __builtin_sqrt(gamma2);
@@ -112,7 +133,7 @@ __global__ void pack_mixed_kernel(size_t n, int n_points, const double* __restri
     r.sv = (float)(scale / gamma2);
     store_rec(out, n_points, p, g - p * (size_t)n_points, r);
     // the FP32 transverse beta needs |beta_x| + |beta_y| << 1 (see RecMixed)
-    if (__builtin_fabs(bx) + __builtin_fabs(by) > 0.25) *off_axis = 1;
+    if (__builtin_fabs(bx) + __builtin_fabs(by) > 0.25) atomicOr(flags, kFlagOffAxis);
 }
 
 // ---------------------------------------------------------------------------

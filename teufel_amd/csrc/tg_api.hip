@@ -216,29 +216,30 @@ int ensure_records(tg_context* c, int prec)
     }
     if (n) {
         unsigned blocks = (unsigned)((n + 255) / 256);
+        if (!c->d_flag) TG_HIP(c, hipMalloc((void**)&c->d_flag, sizeof(int)));
+        TG_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
         if (prec == TG_FP64)
             pack_f64_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
                                                          raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
                                                          raw(c, 8), raw(c, 9), raw_charge(c),
-                                                         (char*)c->d_rec[prec]);
+                                                         (char*)c->d_rec[prec], c->d_flag);
         else
-        {
-            if (!c->d_flag) TG_HIP(c, hipMalloc((void**)&c->d_flag, sizeof(int)));
-            TG_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
             pack_mixed_kernel<<<blocks, 256, 0, c->stream>>>(n, c->n_points, raw(c, 0), raw(c, 1), raw(c, 2),
                                                            raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7),
                                                            raw(c, 8), raw(c, 9), raw_charge(c),
                                                            (char*)c->d_rec[prec], c->d_flag);
-            TG_HIP(c, hipGetLastError());
-            int off_axis = 0;
-            TG_HIP(c, hipMemcpyAsync(&off_axis, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-            TG_HIP(c, hipStreamSynchronize(c->stream));
-            if (off_axis)
-                return fail(c, TG_ERR_ARG,
-                            "TG_MIXED needs beams travelling close to the z axis (|beta_x|+|beta_y| <= 0.25); "
-                            "use TG_FP64");
-        }
         TG_HIP(c, hipGetLastError());
+        int flags = 0;
+        TG_HIP(c, hipMemcpyAsync(&flags, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(c, hipStreamSynchronize(c->stream));
+        if (flags & kFlagSuperluminal)
+            return fail(c, TG_ERR_ARG,
+                        "a trajectory step moves faster than light (|dX| > c dt) or back in time: the retarded time "
+                        "must increase along every trajectory");
+        if (prec == TG_MIXED && (flags & kFlagOffAxis))
+            return fail(c, TG_ERR_ARG,
+                        "TG_MIXED needs beams travelling close to the z axis (|beta_x|+|beta_y| <= 0.25); "
+                        "use TG_FP64");
     }
     c->rec_valid[prec] = true;
     return TG_OK;
