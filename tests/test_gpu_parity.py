@@ -525,16 +525,17 @@ def test_unphysical_trajectories_are_rejected():
     obs, _, _ = W.grid_screen(4, 0.02, 10.0)
     t0 = np.linalg.norm(obs, axis=1) / W.C_LIGHT - 1e-15
 
-    def variant(**change):
+    def variant(change=()):
+        """A copy of base with entries (array name, particle, point) set."""
         tr = oracle.Trajectories(**{nm: getattr(base, nm).copy() for nm in oracle.Trajectories.NAMES},
                                  charge=base.charge.copy())
-        for (name, p, j), v in change.items():
+        for name, p, j, v in change:
             getattr(tr, name)[p, j] = v
         return tr
 
     cases = {
-        "superluminal": variant(**{("x", 1, 3): 0.5}),        # a 0.5 m jump within 1 ps
-        "backward": variant(**{("t", 2, 2): 0.5e-12, ("t", 2, 3): 0.4e-12}),
+        "superluminal": variant([("x", 1, 3, 0.5)]),          # a 0.5 m jump within 1 ps
+        "backward": variant([("t", 2, 2, 0.5e-12), ("t", 2, 3, 0.4e-12)]),
     }
     with teufel_amd.RadiationContext(0) as c:
         sid = c.add_screen(obs, t0, 1e-15, 64)
@@ -544,7 +545,7 @@ def test_unphysical_trajectories_are_rejected():
                 with pytest.raises(teufel_amd.TgError, match="faster than light"):
                     c.integrate(prec)
         # repeated points: a particle resting for a step (dt = 0, dX = 0)
-        rest = variant(**{("t", 0, 4): 3e-12, ("z", 0, 4): 3 * 2.99e-4})
+        rest = variant([("t", 0, 4, 3e-12), ("z", 0, 4, 3 * 2.99e-4)])
         c.set_trajectories(rest)
         c.zero_screen(sid)
         c.integrate(TG_FP64)
