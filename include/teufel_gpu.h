@@ -110,7 +110,12 @@ int tg_screen_count(const tg_context* ctx, int* n_screens);
 /* Observer::integrate() (src/observer.h:140) for every screen of the context,
  * in one batched launch: adds the fields of all particles to every pixel's
  * trace.  precision: TG_FP64 or TG_MIXED.  tuning may be NULL.  Returns after
- * the work is queued on the context's stream; tg_synchronize waits for it. */
+ * the work is queued on the context's stream; tg_synchronize waits for it.
+ * TG_ERR_ARG when a trajectory step moves faster than light or back in time
+ * (the retarded time must increase along every trajectory, as it does for
+ * any particle slower than light), and for TG_MIXED when the beam is far off
+ * the z axis (|beta_x|+|beta_y| > 0.25); both are checked once per
+ * trajectory set and precision. */
 int tg_integrate(tg_context* ctx, int precision, const tg_tuning* tuning);
 int tg_synchronize(tg_context* ctx);
 
