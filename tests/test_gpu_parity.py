@@ -560,3 +560,67 @@ def test_unphysical_trajectories_are_rejected():
         with pytest.raises(teufel_amd.TgError, match="close to the z axis"):
             c.integrate(TG_MIXED)
         c.integrate(TG_FP64)
+
+
+def test_electrostatic_single_time_field():
+    """The reference's tests/teufel.electrostatic.cpp through tg_fields_at
+    (ChargedParticle::RetardedField(t, x), src/particle.cpp:612-769), with its
+    closed forms and tolerances:
+    * -1 pC at rest (a one-point trajectory): the Coulomb field,
+      8987.55 V/m at 1 mm, along +-x (1e-3 V/m) and along z (1e-6 relative);
+    * the same charge at gamma = 10 in direction (2, 3, 5), pushed through a
+      field-free region for 4 ns (2000 Vay steps): at t = 0, 1 mm across the
+      direction of motion, before the trajectory's light cone (the back-
+      extrapolation), E and B of the uniformly moving charge,
+      gamma E_Coulomb across and v x E / c^2 (1e-5 relative); at t = 3 ns on
+      the trajectory (the interpolation and its refinement) the same within
+      1e-3, and the transverse E along 11 points of the line of motion equal
+      to its values at t = 0 (1e-5 relative);
+    and every value against the oracle (1e-12 of the field)."""
+    say("electrostatic test case: Coulomb at rest, uniformly moving charge at t = 0 and t = 3 ns")
+    e_charge, eps0, c_light = 1.6021766208e-19, 8.854187817e-12, 2.99792458e8
+    mu0 = 4e-7 * math.pi
+    q = -1.0e-12 / e_charge
+    e_ref = 1e-12 / (4.0 * math.pi * eps0 * 1.0e-3 * 1.0e-3)
+    with teufel_amd.RadiationContext(0) as c:
+        rest = oracle.Trajectories.empty(1, 1, q)
+        c.set_trajectories(rest)
+        pts = np.array([[1e-3, 0.0, 0.0], [-1e-3, 0.0, 0.0], [0.0, 0.0, 1e-3]])
+        f = c.fields_at(np.zeros(3), pts)
+        assert np.linalg.norm(f[0, :3] - [-e_ref, 0, 0]) < 1e-3
+        assert np.linalg.norm(f[1, :3] - [e_ref, 0, 0]) < 1e-3
+        assert np.linalg.norm(f[2, :3] - [0, 0, -e_ref]) / e_ref < 1e-6
+        for i in range(3):
+            assert np.linalg.norm(f[i] - oracle.field_at(rest, 0, 0.0, pts[i])) <= 1e-12 * e_ref
+
+        gamma = 10.0
+        beta = math.sqrt(1.0 - 1.0 / (gamma * gamma))
+        bg = math.sqrt(gamma * gamma - 1.0)
+        d = np.array([2.0, 3.0, 5.0]) / math.sqrt(38.0)
+        perp = np.cross(d, [1.0, 0.0, 0.0])
+        perp /= np.linalg.norm(perp)
+        c.track_undulator([[0.0, 0.0, 0.0]], [d * bg], 4.0e-9 / 2000, 2000, 0.0, 0.3, 1, [0.0, 0.0, 0.0],
+                          charge=q, mass=1.0)
+        tr = oracle.Trajectories(**c.get_trajectories())
+        e_v = perp * e_ref * -1.0 * gamma
+        b_v = np.cross(d * beta * c_light, perp) * -1.0e-12 * mu0 / (4 * math.pi) / (1e-3 * 1e-3) * gamma
+
+        def rel(a, b):
+            return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+        o0 = perp * 1e-3
+        t_obs = 3.0e-9
+        oT = d * t_obs * beta * c_light + perp * 1e-3
+        line = [-0.005 + i * 0.001 for i in range(11)]
+        times = np.array([0.0, t_obs] + [0.0] * 11 + [t_obs] * 11)
+        pts = np.array([o0, oT] + [o0 + d * l for l in line] + [oT + d * l for l in line])
+        f = c.fields_at(times, pts)
+        assert rel(f[0, :3], e_v) < 1e-5 and rel(f[0, 3:], b_v) < 1e-5
+        assert rel(f[1, :3], e_v) < 1e-3 and rel(f[1, 3:], b_v) < 1e-3
+        tr0 = f[2:13, :3] @ perp
+        trT = f[13:24, :3] @ perp
+        assert np.all((trT - tr0) / tr0 <= 1e-5)
+        for i in range(len(times)):
+            ref = oracle.field_at(tr, 0, times[i], pts[i])
+            assert np.linalg.norm(f[i, :3] - ref[:3]) <= 1e-12 * np.linalg.norm(ref[:3])
+            assert np.linalg.norm(f[i, 3:] - ref[3:]) <= 1e-12 * np.linalg.norm(ref[3:])
