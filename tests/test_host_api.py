@@ -108,6 +108,38 @@ def test_screen_file_reader():
     assert r.stdout.rstrip().endswith("0.000000 0.000000 -1.000000")
 
 
+H5_BACK = os.path.join(ROOT, "tests", "golden", "BackwardDiffractionScreen.h5")
+
+
+def test_backward_screen_file_reader():
+    """The reader on the reference's second screen file
+    (tests/BackwardDiffractionScreen.h5, kept as a fixture): a flat mesh of
+    1338 cells facing +z, 500 bins of 0.2 ps, every observation position at
+    its triangle's centroid, and the values the HDF5 library's h5dump reads
+    from the same file."""
+    _program()
+    r = subprocess.run([BIN, "h5info", H5_BACK], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    v = r.stdout.split()
+    info = dict(zip(v[0::2], v[1::2]))
+    if _h5dump():
+        t0 = _h5dump_values(H5_BACK, "/ObservationTime")
+        total = 0.0
+        for x in t0:
+            total += x
+        assert float(info["t0sum"]) == total
+        assert float(info["t0_0"]) == t0[0]
+        cp = _h5dump_values(H5_BACK, "/MeshCornerPoints")
+        tail = r.stdout.split("cp_last")[1].split()[:3]
+        assert [float(x) for x in tail] == list(cp[-3:])
+        tri = _h5dump_values(H5_BACK, "/MeshTriangles")
+        assert float(info["tri_last"]) == tri[-1]
+    assert info["Ncp"] == "712" and info["Np"] == "1338" and info["Nt"] == "500"
+    assert float(info["dt"]) == 2e-13
+    assert float(info["centroid_dev"]) < 1e-15
+    assert r.stdout.rstrip().endswith("0.000000 0.000000 1.000000")
+
+
 def test_screen_file_writer(tmp_path):
     """MeshedScreen::writeFile (src/mesh.cpp:427-736 layout): the written file
     reads back identically, and the HDF5 library's h5dump reads its
@@ -253,6 +285,38 @@ def test_diffraction_screen_end_to_end(tmp_path, precision):
         assert trace_rel_error(traces[pick], ref) <= 1e-12
     else:
         n = np.array([0.0, 0.0, -1.0])
+        assert energy_rel_error(pixel_energy(traces[pick], dt, n), pixel_energy(ref, dt, n))[0] <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_backward_diffraction_screen(tmp_path, precision):
+    """The reference's BackwardDiffractionScreen.h5 (a screen facing +z)
+    read into a MeshedScreen<Bunch> with the DiffractionScreen test's charge
+    (100 pC at gamma 20 over 20 m), integrated on the GPU and compared on
+    every 5th pixel with the oracle: FP64 within 1e-12, mixed within eps_W
+    1e-5.  The straight trajectory sweeps its field past this screen, so the
+    energy through it is ~0 (the traces are not)."""
+    tr = diffraction_trajectory()
+    traj = str(tmp_path / "traj.bin")
+    with open(traj, "wb") as f:
+        f.write(struct.pack("<ii", 1, tr.n_points))
+        for name in oracle.Trajectories.NAMES:
+            f.write(np.ascontiguousarray(getattr(tr, name), "<f8").tobytes())
+        f.write(np.ascontiguousarray(tr.charge, "<f8").tobytes())
+    dump = str(tmp_path / "backward.dump")
+    r = run("meshscreen", H5_BACK, traj, dump, precision)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.splitlines()[0])
+    pts, t0, dt, traces = read_dump(dump)
+    assert traces.shape == (1338, 500, 6) and dt == 2e-13
+    pick = np.arange(0, 1338, 5)
+    ref = oracle.integrate_screen(tr, pts[pick], t0[pick], dt, 500)
+    assert np.abs(ref).max() > 0
+    if precision == "fp64":
+        assert trace_rel_error(traces[pick], ref) <= 1e-12
+    else:
+        n = np.array([0.0, 0.0, 1.0])
         assert energy_rel_error(pixel_energy(traces[pick], dt, n), pixel_energy(ref, dt, n))[0] <= 1e-5
 
 
