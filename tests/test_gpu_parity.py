@@ -624,3 +624,43 @@ def test_electrostatic_single_time_field():
             ref = oracle.field_at(tr, 0, times[i], pts[i])
             assert np.linalg.norm(f[i, :3] - ref[:3]) <= 1e-12 * np.linalg.norm(ref[:3])
             assert np.linalg.norm(f[i, 3:] - ref[3:]) <= 1e-12 * np.linalg.norm(ref[3:])
+
+
+def test_undulator_trajectory_closed_forms():
+    """The reference's tests/teufel.undulator.cpp through tg_track_undulator:
+    one electron at gamma = 30.358 from the origin along z through a planar
+    undulator of 54 periods of 37 mm at 0.532 T centred at z = 1.5 m, 3000
+    Vay steps over the time that brings it to z = 3.0 m (the slippage of one
+    radiation wavelength per period).  With the reference test's tolerances:
+    the final z within 1e-5 m of 3.0 m, the largest x inside the undulator
+    within 1e-5 m of e B/(beta gamma m c k_u^2), the final time within one
+    step of that time, the final x within 1e-5 m of 0; and the positions
+    within 1e-12 m of the oracle's pusher."""
+    say("undulator test case: 54 x 37 mm at 0.532 T, gamma 30.358, 3000 steps")
+    e_charge, c_light = 1.6021766208e-19, 2.99792458e8
+    B, lam, N = 0.532, 0.037, 54
+    ku = 2 * math.pi / lam
+    gamma = 30.35823388162631
+    beta = math.sqrt(1.0 - 1.0 / (gamma * gamma))
+    bg = math.sqrt(gamma * gamma - 1.0)
+    K = 0.934 * 100 * lam * B
+    lam_r = (lam / (2 * gamma * gamma)) * (1 + K * K / 2)
+    xdis = e_charge * B * c_light / (bg * 9.1e-31 * c_light * c_light * ku * ku)
+    tau = N * (lam + lam_r) / c_light + (3.0 - N * lam) / (beta * c_light)
+    steps = 3000
+    with teufel_amd.RadiationContext(0) as c:
+        c.track_undulator([[0.0, 0.0, 0.0]], [[0.0, 0.0, bg]], tau / steps, steps, B, lam, N, [0.0, 0.0, 1.5],
+                          charge=-1.0, mass=1.0)
+        tr = c.get_trajectories()
+    x, z, t = tr["x"][0], tr["z"][0], tr["t"][0]
+    inside = (z > lam) & (z < N * lam - lam)
+    assert abs(z[-1] - 3.0) <= 1e-5
+    assert abs(x[inside].max() - xdis) <= 1e-5
+    assert abs(t[-1] - tau) <= tau / steps
+    assert abs(x[-1]) <= 1e-5
+    ref = oracle.track_undulator([[0.0, 0.0, 0.0]], [[0.0, 0.0, bg]], tau / steps, steps, B, lam, N, [0.0, 0.0, 1.5],
+                                 charge=-1.0, mass=1.0)
+    # positions within 1e-12 m, as test_gpu_pusher_matches_oracle (the
+    # undulator field's sin/cos/cosh differ from the host's by rounding)
+    assert np.abs(x - ref.x[0]).max() <= 1e-12
+    assert np.abs(z - ref.z[0]).max() <= 1e-12
