@@ -616,7 +616,7 @@ __device__ __forceinline__ void deposit_lane(const Window<float>& win, LaneAcc<f
 // current record from LDS.  The wait for a chunk falls once per 16 steps.
 //
 // ScalarPrefetch (mixed kernel, 64-B records): the record's four 16-B pieces
-// (blocked layout) read with scalar loads, requested one step ahead of use.
+// (piece-major layout) read with scalar loads, requested one step ahead of use.
 constexpr int kChunkRecs = 16;
 constexpr int kChunkDoubles = kChunkRecs * 12;   // 192 doubles = 64 lanes x 3
 
@@ -960,7 +960,7 @@ __device__ __forceinline__ int walk_lanes(const char* __restrict__ rp, int k, in
     return steps;
 }
 
-// The records of particle p (blocked layout).
+// The records of particle p (piece-major layout, rec_piece_offset).
 template <int PREC>
 __device__ __forceinline__ const char* particle_records(const typename RecOf<PREC>::type* recs, int p, int NP)
 {
