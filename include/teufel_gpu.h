@@ -225,6 +225,18 @@ int tg_track_undulator(tg_context* ctx, int n_particles, const double* x0, const
                        double t_start, double charge, double mass, double dt, int n_steps,
                        double bpeak, double lambda_u, int n_periods, const double* origin);
 
+/* The same pusher through a lattice of the planar undulator and a
+ * homogeneous field: Lattice::Field (src/fields.cpp:229-239) summing the
+ * undulator's field and then HomogeneousField(E0, B0) (src/fields.cpp:
+ * 155-163).  origin == NULL: no undulator (bpeak, lambda_u, n_periods
+ * unused).  E0 [V/m], B0 [T] (3 doubles each): the homogeneous field; a NULL
+ * one is zero, both NULL: no homogeneous element.  tg_track_undulator is the
+ * case E0 = B0 = NULL. */
+int tg_track_lattice(tg_context* ctx, int n_particles, const double* x0, const double* u0,
+                     double t_start, double charge, double mass, double dt, int n_steps,
+                     double bpeak, double lambda_u, int n_periods, const double* origin,
+                     const double* E0, const double* B0);
+
 /* Copy particles [p_begin, p_end) of the context's trajectories to host SoA
  * arrays ((p_end-p_begin)*n_points each; any pointer may be NULL). */
 int tg_get_trajectories(tg_context* ctx, int p_begin, int p_end,

@@ -51,6 +51,7 @@ def lib():
         L.or_total_energy.restype = d
         L.or_undulator_field.argtypes = [d, d, i, _dp, _dp, _dp]
         L.or_track_undulator.argtypes = [i, _dp, _dp, d, d, d, d, i, d, d, i, _dp] + [_dp] * 10 + [i]
+        L.or_track_lattice.argtypes = [i, _dp, _dp, d, d, d, d, i, i, d, d, i, _dp, i, _dp, _dp] + [_dp] * 10 + [i]
         L.or_retarded_field.argtypes = [ctypes.c_void_p, i, i, _dp, _dp]
         L.or_retarded_time.argtypes = [ctypes.c_void_p, i, i, _dp]
         L.or_count_in_window.argtypes = [ctypes.c_void_p, i, _dp, _dp, d, i]
@@ -75,6 +76,7 @@ def ref_lib():
         R.ref_pseudo_integrate.argtypes = [i, i, _dp, _dp, d, d, i, _dp]
         R.ref_track_undulator.argtypes = [i, _dp, _dp, d, d, d, d, i, d, d, i, _dp] + [_dp] * 10
         R.ref_track_homogeneous.argtypes = [_dp, _dp, _dp, _dp, d, d, d, i] + [_dp] * 10
+        R.ref_track_lattice.argtypes = [i, _dp, _dp, d, d, d, d, i, i, d, d, i, _dp, i, _dp, _dp] + [_dp] * 10
         R.ref_retarded_field_at.argtypes = [vp, i, d, _dp, _dp]
         _ref = R
     return _ref
@@ -253,6 +255,46 @@ def track_undulator(x0, u0, dt, n_steps, bpeak, lambda_u, n_periods, origin,
     else:
         lib().or_track_undulator(n, _ptr(x0), _ptr(u0), t_start, charge, mass, dt, n_steps, bpeak,
                                  lambda_u, n_periods, _ptr(o), *outs, n_threads)
+    return tr
+
+
+def track_lattice(x0, u0, dt, n_steps, undulator=None, E0=None, B0=None,
+                  charge=-1.0, mass=1.0, t_start=0.0, n_threads=0, use_ref=False):
+    """Vay pusher through a lattice of an undulator (bpeak, lambda_u,
+    n_periods, origin) and a homogeneous field E0 [V/m], B0 [T]; either may
+    be None (oracle restatement, or the reference's own InitVay/StepVay and
+    Lattice with use_ref=True)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1, 3)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1, 3)
+    n = x0.shape[0]
+    tr = Trajectories.empty(n, n_steps + 1, charge)
+    bpeak, lambda_u, n_periods, origin = undulator if undulator is not None else (0.0, 1.0, 0, (0.0, 0.0, 0.0))
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    has_hom = E0 is not None or B0 is not None
+    e = np.ascontiguousarray(np.zeros(3) if E0 is None else E0, dtype=np.float64)
+    b = np.ascontiguousarray(np.zeros(3) if B0 is None else B0, dtype=np.float64)
+    args = (n, _ptr(x0), _ptr(u0), t_start, charge, mass, dt, n_steps, int(undulator is not None),
+            bpeak, lambda_u, n_periods, _ptr(o), int(has_hom), _ptr(e), _ptr(b), *[_ptr(a) for a in tr.arrays()])
+    if use_ref:
+        R = ref_lib()
+        if R is None:
+            raise RuntimeError("oracle/_ref/libteufel_ref.so not built")
+        R.ref_track_lattice(*args)
+    else:
+        lib().or_track_lattice(*args, n_threads)
+    return tr
+
+
+def ref_track_homogeneous(E0, B0, x0, u0, dt, n_steps, charge=-1.0, mass=1.0):
+    """The reference's own InitVay/StepVay in its HomogeneousField (one
+    particle starting at t = 0; needs oracle/_ref)."""
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref/libteufel_ref.so not built")
+    tr = Trajectories.empty(1, n_steps + 1, charge)
+    args = [np.ascontiguousarray(a, dtype=np.float64).reshape(3) for a in (E0, B0, x0, u0)]
+    R.ref_track_homogeneous(*[_ptr(a) for a in args], charge, mass, dt, n_steps,
+                            *[_ptr(a) for a in tr.arrays()])
     return tr
 
 

@@ -434,12 +434,32 @@ static void vay_second_half(vay_state* s, vec3 E, vec3 B)
     s->p_i1 = v_div(v_add(v_add(p_prime, v_mul(tt, v_dot(p_prime, tt))), v_cross(p_prime, tt)), 1 + v_abs2nd(tt));
 }
 
-void or_track_undulator(int n_particles, const double* x0, const double* u0, double t_start,
-                        double charge, double mass, double dt, int n_steps,
-                        double bpeak, double lambda_u, int n_periods, const double origin[3],
-                        double* t, double* x, double* y, double* z,
-                        double* ux, double* uy, double* uz,
-                        double* ax, double* ay, double* az, int n_threads)
+/* Lattice::Field (fields.cpp:229-239): the elements' fields summed in order,
+ * the undulator (has_und) and then a HomogeneousField E0, B0 (has_hom,
+ * fields.cpp:155-163).  Without the homogeneous element the undulator's B is
+ * taken as it is and E is zero. */
+static void lattice_field(int has_und, double bpeak, double lambda_u, int n_periods, const double origin[3],
+                          int has_hom, const double E0[3], const double B0[3], vec3 X, vec3* E, vec3* B)
+{
+    double Xa[3] = {X.x, X.y, X.z}, Ba[3] = {0.0, 0.0, 0.0};
+    if (has_und) or_undulator_field(bpeak, lambda_u, n_periods, origin, Xa, Ba);
+    *E = v_make(0.0, 0.0, 0.0);
+    *B = v_make(Ba[0], Ba[1], Ba[2]);
+    if (has_hom) {
+        *E = v_add(*E, v_make(E0[0], E0[1], E0[2]));
+        *B = v_add(*B, v_make(B0[0], B0[1], B0[2]));
+    }
+}
+
+/* InitVay + StepVay (particle.cpp:407-442, 479-512) through the lattice of
+ * lattice_field; n_steps+1 points per particle, SoA particle-major. */
+void or_track_lattice(int n_particles, const double* x0, const double* u0, double t_start,
+                      double charge, double mass, double dt, int n_steps,
+                      int has_und, double bpeak, double lambda_u, int n_periods, const double origin[3],
+                      int has_hom, const double E0[3], const double B0[3],
+                      double* t, double* x, double* y, double* z,
+                      double* ux, double* uy, double* uz,
+                      double* ax, double* ay, double* az, int n_threads)
 {
     size_t NP = (size_t)n_steps + 1;
 #ifdef _OPENMP
@@ -457,9 +477,8 @@ void or_track_undulator(int n_particles, const double* x0, const double* u0, dou
         s.dt = dt;
         s.qm = charge / mass * OR_INV_REST_MASS;
         s.qmt2 = s.qm * 0.5 * dt;
-        double Xa[3] = {s.X.x, s.X.y, s.X.z}, Ba[3];
-        or_undulator_field(bpeak, lambda_u, n_periods, origin, Xa, Ba);
-        vec3 E = v_make(0.0, 0.0, 0.0), B = v_make(Ba[0], Ba[1], Ba[2]);
+        vec3 E, B;
+        lattice_field(has_und, bpeak, lambda_u, n_periods, origin, has_hom, E0, B0, s.X, &E, &B);
         vay_second_half(&s, E, B);
         double gamma_h = sqrt(v_abs2nd(s.P) + 1.0);
         vec3 beta_h = v_div(s.P, gamma_h);
@@ -473,9 +492,7 @@ void or_track_undulator(int n_particles, const double* x0, const double* u0, dou
             vec3 beta_i = v_div(p_i, s.gamma_i1);
             s.t += s.dt;
             s.X = v_add(s.X, v_mul(v_mul(beta_i, OR_SPEED_OF_LIGHT), s.dt));
-            Xa[0] = s.X.x; Xa[1] = s.X.y; Xa[2] = s.X.z;
-            or_undulator_field(bpeak, lambda_u, n_periods, origin, Xa, Ba);
-            B = v_make(Ba[0], Ba[1], Ba[2]);
+            lattice_field(has_und, bpeak, lambda_u, n_periods, origin, has_hom, E0, B0, s.X, &E, &B);
             s.P = v_add(s.p_i1, v_mul(v_add(v_div(E, OR_SPEED_OF_LIGHT), v_cross(beta_i, B)), s.qmt2));
             gamma_h = sqrt(v_abs2nd(s.P) + 1.0);
             beta_h = v_div(s.P, gamma_h);
@@ -488,4 +505,16 @@ void or_track_undulator(int n_particles, const double* x0, const double* u0, dou
         }
     }
     (void)n_threads;
+}
+
+void or_track_undulator(int n_particles, const double* x0, const double* u0, double t_start,
+                        double charge, double mass, double dt, int n_steps,
+                        double bpeak, double lambda_u, int n_periods, const double origin[3],
+                        double* t, double* x, double* y, double* z,
+                        double* ux, double* uy, double* uz,
+                        double* ax, double* ay, double* az, int n_threads)
+{
+    const double zero[3] = {0.0, 0.0, 0.0};
+    or_track_lattice(n_particles, x0, u0, t_start, charge, mass, dt, n_steps, 1, bpeak, lambda_u, n_periods,
+                     origin, 0, zero, zero, t, x, y, z, ux, uy, uz, ax, ay, az, n_threads);
 }

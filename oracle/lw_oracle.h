@@ -129,6 +129,18 @@ void or_track_undulator(int n_particles, const double* x0, const double* u0, dou
                         double* ux, double* uy, double* uz,
                         double* ax, double* ay, double* az, int n_threads);
 
+/* The same pusher through a lattice (Lattice::Field, src/fields.cpp:229-239)
+ * of the undulator (has_und) and a homogeneous field E0 [V/m], B0 [T]
+ * (has_hom; HomogeneousField, src/fields.cpp:155-163), summed in that order.
+ * or_track_undulator is the case has_und = 1, has_hom = 0. */
+void or_track_lattice(int n_particles, const double* x0, const double* u0, double t_start,
+                      double charge, double mass, double dt, int n_steps,
+                      int has_und, double bpeak, double lambda_u, int n_periods, const double origin[3],
+                      int has_hom, const double E0[3], const double B0[3],
+                      double* t, double* x, double* y, double* z,
+                      double* ux, double* uy, double* uz,
+                      double* ax, double* ay, double* az, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -223,6 +223,35 @@ void ref_track_homogeneous(const double* E, const double* B, const double* x0, c
     }
 }
 
+/* Reference InitVay + StepVay through the reference's Lattice (src/fields.cpp:
+ * 194-239) holding the undulator (has_und) and a HomogeneousField E0, B0
+ * (has_hom), added in that order; the Lattice owns and deletes both. */
+void ref_track_lattice(int n_particles, const double* x0, const double* u0, double t_start, double charge,
+                       double mass, double dt, int n_steps, int has_und, double bpeak, double lambda_u,
+                       int n_periods, const double* origin, int has_hom, const double* E0, const double* B0,
+                       double* t, double* x, double* y, double* z, double* ux, double* uy, double* uz,
+                       double* ax, double* ay, double* az)
+{
+    Lattice lattice;
+    if (has_und) lattice.addElement(new UndulatorField(bpeak, lambda_u, n_periods, Vector(origin[0], origin[1], origin[2])));
+    if (has_hom) lattice.addElement(new HomogeneousField(Vector(E0[0], E0[1], E0[2]), Vector(B0[0], B0[1], B0[2])));
+    for (int p = 0; p < n_particles; p++) {
+        ChargedParticle q(charge, mass);
+        q.initTrajectory(t_start, Vector(x0[3 * p], x0[3 * p + 1], x0[3 * p + 2]),
+                         Vector(u0[3 * p], u0[3 * p + 1], u0[3 * p + 2]), Vector(0.0, 0.0, 0.0));
+        q.InitVay(dt, &lattice);
+        for (int n = 0; n < n_steps; n++) q.StepVay(&lattice);
+        std::size_t g = (std::size_t)p * (n_steps + 1);
+        for (int k = 0; k <= n_steps; k++) {
+            Vector X = q.TrajPoint(k), P = q.TrajMomentum(k), A = q.TrajAccel(k);
+            t[g + k] = q.TrajTime(k);
+            x[g + k] = X.x; y[g + k] = X.y; z[g + k] = X.z;
+            ux[g + k] = P.x; uy[g + k] = P.y; uz[g + k] = P.z;
+            ax[g + k] = A.x; ay[g + k] = A.y; az[g + k] = A.z;
+        }
+    }
+}
+
 /* ChargedParticle::RetardedField(double t, Vector x) (src/particle.cpp:612-769)
  * of particle p of a beam: the single-time light-cone path. */
 void ref_retarded_field_at(void* beam, int p, double obs_time, const double* obs, double* f)

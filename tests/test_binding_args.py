@@ -102,4 +102,10 @@ def test_track_and_spectra_reject_bad_vectors():
     with pytest.raises(ValueError):
         c.track_undulator(np.zeros((4, 3)), np.zeros((4, 3)), 1e-12, 10, 0.03, 0.3, 8, (0, 0))
     with pytest.raises(ValueError):
+        c.track_lattice(np.zeros((4, 3)), np.zeros((4, 3)), 1e-12, 10, E0=(1.0, 2.0))
+    with pytest.raises(ValueError):
+        c.track_lattice(np.zeros((4, 3)), np.zeros((4, 3)), 1e-12, 10, B0=np.zeros(4))
+    with pytest.raises(ValueError):
+        c.track_lattice(np.zeros((4, 3)), np.zeros((4, 3)), 1e-12, 10, undulator=(0.03, 0.3, 8, (0, 0)))
+    with pytest.raises(ValueError):
         c.screen_spectra(0, np.array([1.0, 0.0]), np.array([0.0, 1.0, 0.0]))

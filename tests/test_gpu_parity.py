@@ -16,6 +16,7 @@ import teufel_amd
 from teufel_amd import TG_FP64, TG_MIXED
 from teufel_amd import workloads as W
 from teufel_amd.metrics import energy_rel_error, trace_rel_error
+from tracking_cases import exb_case, magnet_case
 
 pytestmark = pytest.mark.gpu
 
@@ -664,3 +665,74 @@ def test_undulator_trajectory_closed_forms():
     # undulator field's sin/cos/cosh differ from the host's by rounding)
     assert np.abs(x - ref.x[0]).max() <= 1e-12
     assert np.abs(z - ref.z[0]).max() <= 1e-12
+
+
+def _track_lattice_gpu(x0, u0, dt, n_steps, **kw):
+    with teufel_amd.RadiationContext(0) as c:
+        c.track_lattice(x0, u0, dt, n_steps, **kw)
+        return c.get_trajectories()
+
+
+def test_lattice_pusher_homogeneous_fields():
+    """The reference's tests/teufel.magnet.cpp and tests/teufel.EcrossB.cpp
+    through tg_track_lattice (HomogeneousField, src/fields.cpp:155-163).
+    Magnet: gamma = 10 across 0.1 T, one revolution in 1000 steps; the half-
+    way point 2 R_gyro from the start (1e-5 m), the final time (0.01 ps), the
+    return to the start (1e-5 m), |u| (1e-4).  E x B: gamma = 30.358 in
+    crossed fields that cancel, 3000 steps over 10 ns; the final time (one
+    step), the straight line v t (2e-6 m), |u| (1.5e-3).  Both are the
+    reference tests' own bounds.  Every operation of the pusher in a
+    homogeneous field is correctly rounded, so the trajectories equal the
+    oracle's (which equals the compiled reference's) bitwise."""
+    B, u0, dt, n, r_gyro, tau, bg = magnet_case()
+    say("magnet test case: gamma 10 in |B| = 0.1 T, %d steps" % n)
+    tr = _track_lattice_gpu([[0.0, 0.0, 0.0]], [u0], dt, n, B0=B)
+    X = np.stack([tr["x"][0], tr["y"][0], tr["z"][0]], axis=1)
+    assert abs(0.5 * np.linalg.norm(X[n // 2]) - r_gyro) < 1e-5
+    assert 1e12 * abs(tr["t"][0, -1] - tau) < 0.01
+    assert np.linalg.norm(X[-1]) < 1e-5
+    assert abs(math.sqrt(tr["ux"][0, -1] ** 2 + tr["uy"][0, -1] ** 2 + tr["uz"][0, -1] ** 2) - bg) < 1e-4
+    ref = oracle.track_lattice([[0.0, 0.0, 0.0]], [u0], dt, n, B0=B)
+    for name in oracle.Trajectories.NAMES:
+        assert np.array_equal(tr[name], getattr(ref, name)), name
+
+    E, B, u0, v, dt, n, tau = exb_case()
+    say("E x B test case: gamma 30.358 in crossed fields, %d steps" % n)
+    tr = _track_lattice_gpu([[0.0, 0.0, 0.0]], [u0], dt, n, E0=E, B0=B)
+    assert abs(tr["t"][0, -1] - tau) < dt
+    Xf = np.array([tr["x"][0, -1], tr["y"][0, -1], tr["z"][0, -1]])
+    assert np.abs(Xf - v * tau).max() < 2e-6
+    U = np.stack([tr["ux"][0], tr["uy"][0], tr["uz"][0]], axis=1)
+    assert np.abs(np.linalg.norm(U, axis=1) - np.linalg.norm(u0)).max() < 1.5e-3
+    ref = oracle.track_lattice([[0.0, 0.0, 0.0]], [u0], dt, n, E0=E, B0=B)
+    for name in oracle.Trajectories.NAMES:
+        assert np.array_equal(tr[name], getattr(ref, name)), name
+
+
+def test_lattice_pusher_undulator_with_homogeneous_field():
+    """An undulator and a homogeneous E, B in one lattice (Lattice::Field,
+    src/fields.cpp:229-239, the undulator's field first): 16 electrons of the
+    c2 bunch, 600 steps, against the oracle's pusher within 1e-12 m and
+    1e-12 relative in u (the undulator's sin/cos/cosh differ from the host's
+    by rounding); without the homogeneous element tg_track_lattice equals
+    tg_track_undulator bitwise."""
+    w = W.make("c2", n_particles=16, n_steps=600)
+    u, b = w.und, w.bunch
+    und = (u.bpeak, u.lambda_u, u.n_periods, u.origin)
+    E0, B0 = [3e4, -1e4, 2e3], [1e-3, -2e-3, 5e-4]
+    say("undulator + homogeneous field: 16 electrons, 600 steps")
+    tr = _track_lattice_gpu(b.x0, b.u0, w.dt, w.n_steps, undulator=und, E0=E0, B0=B0)
+    ref = oracle.track_lattice(b.x0, b.u0, w.dt, w.n_steps, undulator=und, E0=E0, B0=B0)
+    plain = oracle.track_undulator(b.x0, b.u0, w.dt, w.n_steps, *und)
+    assert np.abs(ref.x - plain.x).max() > 1e-6          # the homogeneous field moves the beam
+    for name in ("x", "y", "z"):
+        assert np.abs(tr[name] - getattr(ref, name)).max() <= 1e-12, name
+    assert np.array_equal(tr["t"], ref.t)
+    for name in ("ux", "uy", "uz"):
+        assert np.abs(tr[name] - getattr(ref, name)).max() <= 1e-12 * np.abs(ref.uz).max(), name
+    with teufel_amd.RadiationContext(0) as c:
+        c.track_undulator(b.x0, b.u0, w.dt, w.n_steps, *und)
+        a = c.get_trajectories()
+    lat = _track_lattice_gpu(b.x0, b.u0, w.dt, w.n_steps, undulator=und)
+    for name in oracle.Trajectories.NAMES:
+        assert np.array_equal(a[name], lat[name]), name

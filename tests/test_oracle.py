@@ -37,6 +37,11 @@ Each oracle function is checked against something that does not restate it:
                                 curl B = 0 by finite differences
   or_track_undulator            |u| conserved in a magnetic field; peak angle K/gamma;
                                 the reference's own pusher (bitwise)
+  or_track_lattice              the reference's tests/teufel.magnet.cpp (gyro radius,
+                                period, return to the start, |u|) and
+                                tests/teufel.EcrossB.cpp (undeflected drift, |u|)
+                                closed forms; the reference's own pusher in its
+                                HomogeneousField and Lattice (bitwise)
   all of the above              the reference compiled from its sources
                                 (oracle/_ref, this container only), bitwise
 """
@@ -49,6 +54,7 @@ import pytest
 
 import oracle
 from teufel_amd import workloads as W
+from tracking_cases import exb_case, magnet_case
 
 C = 2.99792458e8
 EPS0 = 8.854187817e-12
@@ -483,6 +489,66 @@ def test_pusher_equals_reference():
                                use_ref=True)
     for n in oracle.Trajectories.NAMES:
         assert np.array_equal(getattr(a, n), getattr(r, n)), n
+
+
+def test_lattice_pusher_magnet_closed_forms():
+    B, u0, dt, n, r_gyro, tau, bg = magnet_case()
+    tr = oracle.track_lattice([0.0, 0.0, 0.0], [u0], dt, n, B0=B)
+    X = np.stack([tr.x[0], tr.y[0], tr.z[0]], axis=1)
+    assert abs(0.5 * np.linalg.norm(X[n // 2]) - r_gyro) < 1e-5          # the reference test's bounds
+    assert 1e12 * abs(tr.t[0, -1] - tau) < 0.01
+    assert np.linalg.norm(X[-1]) < 1e-5
+    assert abs(math.sqrt(tr.ux[0, -1] ** 2 + tr.uy[0, -1] ** 2 + tr.uz[0, -1] ** 2) - bg) < 1e-4
+    # the orbit's centre: every point lies r_gyro from it, in the plane normal to B
+    c = 0.5 * X[n // 2]
+    assert np.abs(np.linalg.norm(X - c, axis=1) - r_gyro).max() < 1e-5
+    assert np.abs(X @ B).max() < 1e-12
+
+
+def test_lattice_pusher_exb_closed_forms():
+    E, B, u0, v, dt, n, tau = exb_case()
+    tr = oracle.track_lattice([0.0, 0.0, 0.0], [u0], dt, n, E0=E, B0=B)
+    assert abs(tr.t[0, -1] - tau) < dt
+    X = np.array([tr.x[0, -1], tr.y[0, -1], tr.z[0, -1]])
+    assert np.abs(X - v * tau).max() < 2e-6
+    U = np.stack([tr.ux[0], tr.uy[0], tr.uz[0]], axis=1)
+    assert np.abs(np.linalg.norm(U, axis=1) - np.linalg.norm(u0)).max() < 1.5e-3
+    # no force: d(beta gamma)/dt = e/m_e (beta x B + E/c) cancels to rounding of
+    # its two terms, each of size e |B| / m_e ~ 1.2e11 /s
+    assert np.abs(np.stack([tr.ax, tr.ay, tr.az])).max() < 1e-12 * 1.7588e11 * np.linalg.norm(B)
+
+
+def test_lattice_pusher_undulator_only_is_track_undulator():
+    wb = W.make("c2", n_particles=3, n_steps=400)
+    u, b = wb.und, wb.bunch
+    a = oracle.track_lattice(b.x0, b.u0, wb.dt, wb.n_steps, undulator=(u.bpeak, u.lambda_u, u.n_periods, u.origin))
+    r = oracle.track_undulator(b.x0, b.u0, wb.dt, wb.n_steps, u.bpeak, u.lambda_u, u.n_periods, u.origin)
+    for n in oracle.Trajectories.NAMES:
+        assert np.array_equal(getattr(a, n), getattr(r, n)), n
+
+
+@needs_ref
+def test_lattice_pusher_equals_reference():
+    B, u0, dt, n, *_ = magnet_case()
+    E2, B2, u2, _, dt2, n2, _ = exb_case()
+    wb = W.make("c2", n_particles=3, n_steps=400)
+    u, b = wb.und, wb.bunch
+    und = (u.bpeak, u.lambda_u, u.n_periods, u.origin)
+    cases = [
+        (dict(x0=[0.0, 0.0, 0.0], u0=[u0], dt=dt, n_steps=n, B0=B),
+         oracle.ref_track_homogeneous([0.0, 0.0, 0.0], B, [0.0, 0.0, 0.0], u0, dt, n)),
+        (dict(x0=[0.0, 0.0, 0.0], u0=[u2], dt=dt2, n_steps=n2, E0=E2, B0=B2),
+         oracle.ref_track_homogeneous(E2, B2, [0.0, 0.0, 0.0], u2, dt2, n2)),
+        (dict(x0=b.x0, u0=b.u0, dt=wb.dt, n_steps=wb.n_steps, undulator=und, E0=[3e4, -1e4, 2e3],
+              B0=[1e-3, -2e-3, 5e-4]), None),
+    ]
+    for kw, ref in cases:
+        a = oracle.track_lattice(**kw)
+        r = oracle.track_lattice(**kw, use_ref=True)
+        for name in oracle.Trajectories.NAMES:
+            assert np.array_equal(getattr(a, name), getattr(r, name)), name
+            if ref is not None:
+                assert np.array_equal(getattr(a, name), getattr(ref, name)), name
 
 
 @needs_ref

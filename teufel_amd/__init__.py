@@ -128,6 +128,25 @@ class RadiationContext:
         self._chk(self.lib.tg_track_undulator(self.h, x0.shape[0], _dptr(x0), _dptr(u0), t_start, charge,
                                               mass, dt, n_steps, bpeak, lambda_u, n_periods, _dptr(o)))
 
+    def track_lattice(self, x0, u0, dt, n_steps, undulator=None, E0=None, B0=None,
+                      charge=-1.0, mass=1.0, t_start=0.0):
+        """Vay pusher through a lattice of an undulator (bpeak, lambda_u,
+        n_periods, origin) and a homogeneous field E0 [V/m], B0 [T]; any of
+        the three may be None."""
+        x0 = _f64(x0, (-1, 3))
+        u0 = _check_shape(_f64(u0, (-1, 3)), x0.shape, "u0")
+        if undulator is not None:
+            bpeak, lambda_u, n_periods, origin = undulator
+            oa = _check_shape(_f64(origin).reshape(-1), (3,), "origin")
+            o = _dptr(oa)
+        else:
+            bpeak, lambda_u, n_periods, o = 0.0, 0.0, 0, None
+        e = None if E0 is None else _check_shape(_f64(E0).reshape(-1), (3,), "E0")
+        b = None if B0 is None else _check_shape(_f64(B0).reshape(-1), (3,), "B0")
+        self._chk(self.lib.tg_track_lattice(self.h, x0.shape[0], _dptr(x0), _dptr(u0), t_start, charge, mass, dt,
+                                            n_steps, bpeak, lambda_u, n_periods, o,
+                                            None if e is None else _dptr(e), None if b is None else _dptr(b)))
+
     def trajectory_shape(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         self._chk(self.lib.tg_trajectory_shape(self.h, ctypes.byref(a), ctypes.byref(b)))

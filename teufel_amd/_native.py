@@ -50,6 +50,7 @@ SIGNATURES = [
     ("tg_last_timing", _i, [_vp, _fp, _fp]),
     ("tg_last_stats", _i, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 3),
     ("tg_track_undulator", _i, [_vp, _i, _dp, _dp, _d, _d, _d, _d, _i, _d, _d, _i, _dp]),
+    ("tg_track_lattice", _i, [_vp, _i, _dp, _dp, _d, _d, _d, _d, _i, _d, _d, _i, _dp, _dp, _dp]),
     ("tg_get_trajectories", _i, [_vp, _i, _i] + [_dp] * 11),
     ("tg_trajectory_shape", _i, [_vp, _ip, _ip]),
 ]

@@ -1595,15 +1595,15 @@ int tg_autotune(tg_context* c, int precision, tg_tuning* out)
     return TG_OK;
 }
 
-int tg_track_undulator(tg_context* c, int n_particles, const double* x0, const double* u0, double t_start,
-                       double charge, double mass, double dt, int n_steps, double bpeak, double lambda_u,
-                       int n_periods, const double* origin)
+int tg_track_lattice(tg_context* c, int n_particles, const double* x0, const double* u0, double t_start,
+                     double charge, double mass, double dt, int n_steps, double bpeak, double lambda_u,
+                     int n_periods, const double* origin, const double* E0, const double* B0)
 {
     if (check_ctx(c)) return TG_ERR_ARG;
     if (n_particles < 0 || n_steps < 0) return fail(c, TG_ERR_ARG, "negative size");
     if (n_particles > 0 && (!x0 || !u0)) return fail(c, TG_ERR_ARG, "null initial conditions");
-    if (!origin) return fail(c, TG_ERR_ARG, "null origin");
-    if (!(dt > 0.0) || !(lambda_u > 0.0) || !(mass > 0.0)) return fail(c, TG_ERR_ARG, "dt, lambda_u, mass must be > 0");
+    if (!(dt > 0.0) || !(mass > 0.0)) return fail(c, TG_ERR_ARG, "dt and mass must be > 0");
+    if (origin && !(lambda_u > 0.0)) return fail(c, TG_ERR_ARG, "lambda_u must be > 0");
     TG_HIP(c, hipSetDevice(c->device));
     int rc = alloc_traj(c, n_particles, n_steps + 1);
     if (rc) return rc;
@@ -1617,24 +1617,40 @@ int tg_track_undulator(tg_context* c, int n_particles, const double* x0, const d
     std::vector<double> q((size_t)n_particles, charge);
     if (e == hipSuccess) e = hipMemcpy(raw_charge(c), q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        Undulator u;
-        u.bpeak = bpeak;
-        u.lambda_u = lambda_u;
-        u.kz = 2.0 * kPi / lambda_u;
-        u.z1 = -(n_periods + 1) * lambda_u / 2.0;
-        u.z2 = (n_periods + 1) * lambda_u / 2.0;
-        u.origin = V3{origin[0], origin[1], origin[2]};
+        LatticeField lat{};
+        lat.has_und = origin != nullptr;
+        if (origin) {
+            lat.und.bpeak = bpeak;
+            lat.und.lambda_u = lambda_u;
+            lat.und.kz = 2.0 * kPi / lambda_u;
+            lat.und.z1 = -(n_periods + 1) * lambda_u / 2.0;
+            lat.und.z2 = (n_periods + 1) * lambda_u / 2.0;
+            lat.und.origin = V3{origin[0], origin[1], origin[2]};
+        }
+        lat.has_hom = E0 != nullptr || B0 != nullptr;
+        lat.E0 = E0 ? V3{E0[0], E0[1], E0[2]} : V3{0.0, 0.0, 0.0};
+        lat.B0 = B0 ? V3{B0[0], B0[1], B0[2]} : V3{0.0, 0.0, 0.0};
         double qm = charge / mass * kInvRestMass;
         double qmt2 = qm * 0.5 * dt;
-        vay_undulator_kernel<<<(n_particles + 63) / 64, 64, 0, c->stream>>>(
-            n_particles, d0, d0 + 3 * (size_t)n_particles, t_start, qm, qmt2, dt, n_steps, u, raw(c, 0), raw(c, 1),
+        vay_lattice_kernel<<<(n_particles + 63) / 64, 64, 0, c->stream>>>(
+            n_particles, d0, d0 + 3 * (size_t)n_particles, t_start, qm, qmt2, dt, n_steps, lat, raw(c, 0), raw(c, 1),
             raw(c, 2), raw(c, 3), raw(c, 4), raw(c, 5), raw(c, 6), raw(c, 7), raw(c, 8), raw(c, 9));
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d0);
-    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_track_undulator: ") + hipGetErrorString(e));
+    if (e != hipSuccess) return fail(c, TG_ERR_HIP, std::string("tg_track_lattice: ") + hipGetErrorString(e));
     return TG_OK;
+}
+
+int tg_track_undulator(tg_context* c, int n_particles, const double* x0, const double* u0, double t_start,
+                       double charge, double mass, double dt, int n_steps, double bpeak, double lambda_u,
+                       int n_periods, const double* origin)
+{
+    if (check_ctx(c)) return TG_ERR_ARG;
+    if (!origin) return fail(c, TG_ERR_ARG, "null origin");
+    return tg_track_lattice(c, n_particles, x0, u0, t_start, charge, mass, dt, n_steps, bpeak, lambda_u, n_periods,
+                            origin, nullptr, nullptr);
 }
 
 }  // extern "C"
